@@ -1,0 +1,375 @@
+"""ctypes binding of the C-ABI in include/lbm.h (liblbm_hip.so).
+
+This is the Python view of the product path: every call goes through the
+HIP library.  There is no fallback: if the library was not built, or a GPU
+call fails, an exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "lib", "liblbm_hip.so")
+HEADER = os.path.join(ROOT, "include", "lbm.h")
+
+# enums of include/lbm.h
+OK = 0
+D2Q9, D3Q19, D3Q27 = 9, 19, 27
+STORE_F32, STORE_F64, STORE_F16, STORE_BF16 = 0, 1, 2, 3
+MODE_STANDARD, MODE_REFERENCE = 0, 1
+FLOW_NONE, FLOW_POINT, FLOW_PLANE, FLOW_BOX, FLOW_MASK = 0, 1, 2, 3, 4
+FLAG_INTERNAL, FLAG_REFLECTED = 1, 2
+MASK_SOLID = 1 << 31
+MASK_LID = 1 << 30
+STORAGE_BYTES = {STORE_F32: 4, STORE_F64: 8, STORE_F16: 2, STORE_BF16: 2}
+STORAGE_NAMES = {"f32": STORE_F32, "f64": STORE_F64, "f16": STORE_F16,
+                 "bf16": STORE_BF16}
+
+
+class LbmError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where} failed ({code}): {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("lattice", C.c_int32), ("storage", C.c_int32), ("mode", C.c_int32),
+        ("tau", C.c_float),
+        ("periodic", C.c_int32 * 3),
+        ("lid_face", C.c_int32),
+        ("lid_u", C.c_float * 3),
+        ("wall_rho", C.c_float),
+        ("force", C.c_float * 3),
+    ]
+
+
+class LatticePoint(C.Structure):
+    """The reference's 144-byte LatticePoint record."""
+    _fields_ = [
+        ("x", C.c_int32), ("y", C.c_int32), ("z", C.c_int32),
+        ("particle_distribution", C.c_float * 27),
+        ("isReflected", C.c_uint8), ("isInternal", C.c_uint8),
+        ("pad_", C.c_uint8 * 6),
+        ("reflections", C.c_void_p), ("d_reflections", C.c_void_p),
+    ]
+
+
+assert C.sizeof(LatticePoint) == 144
+
+# numpy view of LatticePoint records
+POINT_DTYPE = np.dtype({
+    "names": ["x", "y", "z", "f", "isReflected", "isInternal", "pad",
+              "reflections", "d_reflections"],
+    "formats": [np.int32, np.int32, np.int32, (np.float32, 27), np.uint8,
+                np.uint8, (np.uint8, 6), np.uint64, np.uint64],
+    "offsets": [0, 4, 8, 12, 120, 121, 122, 128, 136],
+    "itemsize": 144,
+})
+
+P = C.c_void_p
+CTX = C.c_void_p
+i32 = C.c_int32
+i64 = C.c_int64
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "lbm_last_error": (C.c_char_p, []),
+    "lbm_abi_version": (C.c_int, []),
+    "lbm_device_info": (C.c_int, [P, C.c_char_p, C.c_size_t]),
+    "lbm_create": (C.c_int, [C.POINTER(Config), P, C.POINTER(CTX)]),
+    "lbm_destroy": (C.c_int, [CTX]),
+    "lbm_get_config": (C.c_int, [CTX, C.POINTER(Config)]),
+    "lbm_node_count": (i64, [CTX]),
+    "lbm_device_bytes": (i64, [CTX]),
+    "lbm_set_tau": (C.c_int, [CTX, C.c_float]),
+    "lbm_set_body_force": (C.c_int, [CTX, P]),
+    "lbm_synchronize": (C.c_int, [CTX]),
+    "lbm_upload_points": (C.c_int, [CTX, P, i64]),
+    "lbm_download_points": (C.c_int, [CTX, P, i64]),
+    "lbm_set_populations_f32": (C.c_int, [CTX, P]),
+    "lbm_get_populations_f32": (C.c_int, [CTX, P]),
+    "lbm_set_populations_f64": (C.c_int, [CTX, P]),
+    "lbm_get_populations_f64": (C.c_int, [CTX, P]),
+    "lbm_init_equilibrium": (C.c_int, [CTX, P, P]),
+    "lbm_upload_solid": (C.c_int, [CTX, P]),
+    "lbm_get_mask": (C.c_int, [CTX, P]),
+    "lbm_upload_flags": (C.c_int, [CTX, P]),
+    "lbm_get_flags": (C.c_int, [CTX, P]),
+    "lbm_upload_reflections": (C.c_int, [CTX, i32, P, P, P]),
+    "lbm_set_flow": (C.c_int, [CTX, C.c_int, P, P, P]),
+    "lbm_stream": (C.c_int, [CTX]),
+    "lbm_collide": (C.c_int, [CTX, C.c_float]),
+    "lbm_reflect": (C.c_int, [CTX, C.c_int]),
+    "lbm_flow": (C.c_int, [CTX]),
+    "lbm_prime_points": (C.c_int, [CTX]),
+    "lbm_set_force": (C.c_int, [CTX, P]),
+    "lbm_get_force": (C.c_int, [CTX, P]),
+    "lbm_step": (C.c_int, [CTX, C.c_int]),
+    "lbm_step_reference": (C.c_int, [CTX, C.c_int, C.c_int]),
+    "lbm_time_steps": (C.c_int, [CTX, C.c_int, C.POINTER(C.c_float)]),
+    "lbm_time_steps_reference": (C.c_int, [CTX, C.c_int, C.c_int,
+                                           C.POINTER(C.c_float)]),
+    "lbm_moments": (C.c_int, [CTX, P, P]),
+    "lbm_conservation": (C.c_int, [CTX, P]),
+    "lbm_test_coordinates": (C.c_int, [CTX, C.POINTER(C.c_int)]),
+    "lbm_random_spheres": (i64, [i32, i32, i32, C.c_uint64, C.c_double,
+                                 C.c_double, C.c_double, i32, P, i64, P]),
+}
+
+_lib = None
+
+
+def header_symbols() -> list[str]:
+    """Names of the functions declared in include/lbm.h."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lbm_\w+)\s*\(", text)))
+
+
+def load() -> C.CDLL:
+    """Load liblbm_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with "
+            "`python -m latticeboltzmanncuda_amd.build` (there is no "
+            "fallback path)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def check(code: int, where: str) -> None:
+    if code != OK:
+        msg = load().lbm_last_error()
+        raise LbmError(code, where, msg.decode() if msg else "")
+
+
+def device_info() -> tuple[int, str]:
+    n = C.c_int(0)
+    buf = C.create_string_buffer(256)
+    check(load().lbm_device_info(C.byref(n), buf, 256), "lbm_device_info")
+    return n.value, buf.value.decode()
+
+
+def make_config(nx, ny, nz, lattice=D3Q19, storage=STORE_F32,
+                mode=MODE_STANDARD, tau=0.6, periodic=(0, 0, 0), lid_face=-1,
+                lid_u=(0.0, 0.0, 0.0), wall_rho=1.0,
+                force=(0.0, 0.0, 0.0)) -> Config:
+    c = Config()
+    c.nx, c.ny, c.nz = nx, ny, nz
+    c.lattice, c.storage, c.mode = lattice, storage, mode
+    c.tau = tau
+    c.periodic[:] = list(periodic)
+    c.lid_face = lid_face
+    c.lid_u[:] = list(lid_u)
+    c.wall_rho = wall_rho
+    c.force[:] = list(force)
+    return c
+
+
+class Context:
+    """One lattice resident on the GPU (an lbm_ctx handle)."""
+
+    def __init__(self, cfg: Config, stream: int | None = None):
+        self.lib = load()
+        self.cfg = cfg
+        h = CTX()
+        check(self.lib.lbm_create(C.byref(cfg), stream, C.byref(h)),
+              "lbm_create")
+        self.h = h
+        self.q = cfg.lattice
+        self.dims = (cfg.nx, cfg.ny, cfg.nz)
+        self.n = cfg.nx * cfg.ny * cfg.nz
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lbm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- exchange ---------------------------------------------------------
+    def upload_points(self, pts: np.ndarray):
+        assert pts.dtype == POINT_DTYPE and pts.size == self.n
+        check(self.lib.lbm_upload_points(self.h, _ptr(pts), self.n),
+              "lbm_upload_points")
+
+    def download_points(self) -> np.ndarray:
+        pts = np.zeros(self.n, POINT_DTYPE)
+        check(self.lib.lbm_download_points(self.h, _ptr(pts), self.n),
+              "lbm_download_points")
+        return pts
+
+    def set_populations(self, f: np.ndarray):
+        f = np.ascontiguousarray(f)
+        assert f.size == self.q * self.n
+        if f.dtype == np.float64:
+            check(self.lib.lbm_set_populations_f64(self.h, _ptr(f)),
+                  "lbm_set_populations_f64")
+        else:
+            f = np.ascontiguousarray(f, dtype=np.float32)
+            check(self.lib.lbm_set_populations_f32(self.h, _ptr(f)),
+                  "lbm_set_populations_f32")
+
+    def populations(self, dtype=np.float32) -> np.ndarray:
+        out = np.empty(self.q * self.n, dtype)
+        if dtype == np.float64:
+            check(self.lib.lbm_get_populations_f64(self.h, _ptr(out)),
+                  "lbm_get_populations_f64")
+        else:
+            check(self.lib.lbm_get_populations_f32(self.h, _ptr(out)),
+                  "lbm_get_populations_f32")
+        return out
+
+    def init_equilibrium(self, rho=None, u=None):
+        r = None if rho is None else np.ascontiguousarray(rho, np.float64)
+        uu = None if u is None else np.ascontiguousarray(u, np.float64)
+        check(self.lib.lbm_init_equilibrium(self.h, _ptr(r), _ptr(uu)),
+              "lbm_init_equilibrium")
+
+    # -- geometry ---------------------------------------------------------
+    def upload_solid(self, solid):
+        s = None if solid is None else np.ascontiguousarray(solid, np.uint8)
+        check(self.lib.lbm_upload_solid(self.h, _ptr(s)), "lbm_upload_solid")
+
+    def mask(self) -> np.ndarray:
+        m = np.empty(self.n, np.uint32)
+        check(self.lib.lbm_get_mask(self.h, _ptr(m)), "lbm_get_mask")
+        return m
+
+    def upload_flags(self, flags):
+        f = np.ascontiguousarray(flags, np.uint8)
+        check(self.lib.lbm_upload_flags(self.h, _ptr(f)), "lbm_upload_flags")
+
+    def flags(self) -> np.ndarray:
+        f = np.empty(self.n, np.uint8)
+        check(self.lib.lbm_get_flags(self.h, _ptr(f)), "lbm_get_flags")
+        return f
+
+    def upload_reflections(self, node, dirs, weights):
+        node = np.ascontiguousarray(node, np.int32)
+        dirs = np.ascontiguousarray(dirs, np.uint32)
+        weights = np.ascontiguousarray(weights, np.float32)
+        check(self.lib.lbm_upload_reflections(self.h, len(node), _ptr(node),
+                                              _ptr(dirs), _ptr(weights)),
+              "lbm_upload_reflections")
+
+    def set_flow(self, kind, params=None, induced=None, mask=None):
+        p = None if params is None else np.ascontiguousarray(params, np.int32)
+        ind = None if induced is None else np.ascontiguousarray(induced,
+                                                                np.float32)
+        m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        check(self.lib.lbm_set_flow(self.h, kind, _ptr(p), _ptr(ind), _ptr(m)),
+              "lbm_set_flow")
+
+    # -- stages / steps ---------------------------------------------------
+    def stream(self):
+        check(self.lib.lbm_stream(self.h), "lbm_stream")
+
+    def collide(self, tau):
+        check(self.lib.lbm_collide(self.h, tau), "lbm_collide")
+
+    def reflect(self, with_force=False):
+        check(self.lib.lbm_reflect(self.h, int(with_force)), "lbm_reflect")
+
+    def flow(self):
+        check(self.lib.lbm_flow(self.h), "lbm_flow")
+
+    def set_force(self, f):
+        a = np.ascontiguousarray(f, np.float32)
+        check(self.lib.lbm_set_force(self.h, _ptr(a)), "lbm_set_force")
+
+    def force(self) -> np.ndarray:
+        a = np.zeros(3, np.float32)
+        check(self.lib.lbm_get_force(self.h, _ptr(a)), "lbm_get_force")
+        return a
+
+    def step(self, n=1):
+        check(self.lib.lbm_step(self.h, n), "lbm_step")
+
+    def step_reference(self, n=1, with_force=False):
+        check(self.lib.lbm_step_reference(self.h, n, int(with_force)),
+              "lbm_step_reference")
+
+    def time_steps(self, n) -> float:
+        ms = C.c_float(0)
+        check(self.lib.lbm_time_steps(self.h, n, C.byref(ms)),
+              "lbm_time_steps")
+        return ms.value
+
+    def time_steps_reference(self, n, with_force=False) -> float:
+        ms = C.c_float(0)
+        check(self.lib.lbm_time_steps_reference(self.h, n, int(with_force),
+                                                C.byref(ms)),
+              "lbm_time_steps_reference")
+        return ms.value
+
+    def synchronize(self):
+        check(self.lib.lbm_synchronize(self.h), "lbm_synchronize")
+
+    # -- readback ---------------------------------------------------------
+    def moments(self):
+        rho = np.empty(self.n, np.float32)
+        u = np.empty(3 * self.n, np.float32)
+        check(self.lib.lbm_moments(self.h, _ptr(rho), _ptr(u)), "lbm_moments")
+        return rho, u.reshape(3, self.n)
+
+    def conservation(self) -> np.ndarray:
+        out = np.zeros(4, np.float64)
+        check(self.lib.lbm_conservation(self.h, _ptr(out)), "lbm_conservation")
+        return out
+
+    def test_coordinates(self) -> bool:
+        ok = C.c_int(0)
+        check(self.lib.lbm_test_coordinates(self.h, C.byref(ok)),
+              "lbm_test_coordinates")
+        return bool(ok.value)
+
+    def device_bytes(self) -> int:
+        return int(self.lib.lbm_device_bytes(self.h))
+
+
+def random_spheres(dims, seed, frac, rmin=4.0, rmax=8.0, periodic_x=True,
+                   max_spheres=1 << 16):
+    """Solid mask of the random-sphere channel (host setup in the library)."""
+    n = int(np.prod(dims))
+    solid = np.zeros(n, np.uint8)
+    sph = np.zeros(4 * max_spheres, np.float64)
+    ns = load().lbm_random_spheres(dims[0], dims[1], dims[2], seed, frac,
+                                   rmin, rmax, 1 if periodic_x else 0,
+                                   _ptr(solid), max_spheres, _ptr(sph))
+    if ns < 0:
+        check(int(-ns), "lbm_random_spheres")
+    return solid, sph[: 4 * min(ns, max_spheres)].reshape(-1, 4)

@@ -1,0 +1,346 @@
+// Reference-mode kernels: the reference's D3Q27 stages re-designed for the
+// blocked structure-of-arrays layout of lbm_layout.h (node index
+// n = z + nz*(y + ny*x)), one thread per node over a flat grid.
+//
+//   k_ref_stream          SimCalcFuncs.cu:79-96   pull, zero outside the box
+//   k_ref_collide         SimCalcFuncs.cu:98-156  SPLIT velocity, FP64
+//                                                 equilibrium, FP32 relax;
+//                         optionally followed by SimCalcFuncs.cu:200-215
+//                         (flow injection) in the same pass
+//   k_ref_stream_collide  Lattice.cpp:222-226     both of the above fused
+//   k_ref_reflect         SimCalcFuncs.cu:158-198 reflection tables
+//   k_ref_flow            SimCalcFuncs.cu:200-215 flow injection
+//
+// The reference copies the whole lattice before streaming and before
+// reflecting (Lattice.cpp:126-133,176-183).  Here streaming goes A -> B and
+// every population is written, so no copy is needed; reflection runs in
+// place: a reflecting (internal) node reads only its own populations, and
+// the target (x + e_d, d) of a slot has exactly one possible source node
+// x, so the scatter needs no atomics and has a fixed summation order.
+//
+// Floating-point contraction is off in this file so that every operation
+// rounds as written in the reference source.
+#include <hip/hip_runtime.h>
+
+#include "lbm_internal.h"
+#include "lbm_lattice.h"
+#include "lbm_layout.h"
+
+#pragma clang fp contract(off)
+
+namespace lbm {
+namespace {
+
+constexpr int kBlock = 256;
+using L27 = Lat<27>;
+
+__device__ __forceinline__ void decode(const Grid& g, uint32_t n, uint32_t& x,
+                                       uint32_t& y, uint32_t& z) {
+  const uint32_t t = g.dz.div(n);
+  z = n - t * g.nz;
+  x = g.dy.div(t);
+  y = t - x * g.ny;
+}
+
+__device__ __forceinline__ bool flow_match(const FlowSpec& fl, uint32_t n,
+                                           int x, int y, int z) {
+  switch (fl.kind) {
+    case LBM_FLOW_POINT:
+      return x == fl.params[0] && y == fl.params[1] && z == fl.params[2];
+    case LBM_FLOW_PLANE: {
+      const int c = fl.params[0] == 0 ? x : (fl.params[0] == 1 ? y : z);
+      return c == fl.params[1];
+    }
+    case LBM_FLOW_BOX:
+      return x >= fl.params[0] && y >= fl.params[1] && z >= fl.params[2] &&
+             x <= fl.params[3] && y <= fl.params[4] && z <= fl.params[5];
+    case LBM_FLOW_MASK:
+      return fl.mask[n] != 0;
+    default:
+      return false;
+  }
+}
+
+// Zero-fill pull of direction j into f[j]: f_j(x) = A_j(x - e_j).
+__device__ __forceinline__ void pull27(const Grid& g, const float* __restrict__ A,
+                                       uint32_t n, int x, int y, int z,
+                                       float f[27]) {
+#pragma unroll
+  for (int j = 0; j < 27; ++j) {
+    const int sx = x - L27::E[j][0], sy = y - L27::E[j][1],
+              sz = z - L27::E[j][2];
+    const bool in = sx >= 0 && sy >= 0 && sz >= 0 && sx < (int)g.nx &&
+                    sy < (int)g.ny && sz < (int)g.nz;
+    const uint32_t src = in ? (uint32_t)sz + g.nz * ((uint32_t)sy + g.ny * (uint32_t)sx) : n;
+    const float v = A[pidx<27>(src, j)];
+    f[j] = in ? v : 0.0f;
+  }
+}
+
+// SimCalcFuncs.cu:102-155 on one node's 27 populations.
+__device__ __forceinline__ void collide27(float f[27], float tau) {
+  float density = 0.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 27; ++i) {
+    density += f[i];
+    ux += f[i] * (float)L27::E[i][0] * kSplit27[i];
+    uy += f[i] * (float)L27::E[i][1] * kSplit27[i];
+    uz += f[i] * (float)L27::E[i][2] * kSplit27[i];
+  }
+  if (density != 0.0f) {
+    ux /= density;
+    uy /= density;
+    uz /= density;
+  }
+  const float usq = (ux * ux) + (uy * uy) + (uz * uz);
+#pragma unroll
+  for (int i = 0; i < 27; ++i) {
+    const float ct = ((float)L27::E[i][0] * ux) + ((float)L27::E[i][1] * uy) +
+                     ((float)L27::E[i][2] * uz);
+    const double w = i == 0 ? 8.0 / 27.0
+                            : (i < 7 ? 2.0 / 27.0 : (i < 19 ? 1.0 / 54.0 : 1.0 / 216.0));
+    const float eq =
+        (float)(w * density * (1 + (3.0 * ct) + (4.5 * ct * ct) - (1.5 * usq)));
+    f[i] = f[i] + (eq - f[i]) / tau;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_ref_stream(Grid g, const float* __restrict__ A, float* __restrict__ B) {
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n) return;
+  uint32_t x, y, z;
+  decode(g, n, x, y, z);
+  float f[27];
+  pull27(g, A, n, (int)x, (int)y, (int)z, f);
+#pragma unroll
+  for (int j = 0; j < 27; ++j) B[pidx<27>(n, j)] = f[j];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_ref_collide(Grid g, float* __restrict__ F, float tau, FlowSpec flow) {
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n) return;
+  float f[27];
+#pragma unroll
+  for (int i = 0; i < 27; ++i) f[i] = F[pidx<27>(n, i)];
+  collide27(f, tau);
+  if (flow.kind != LBM_FLOW_NONE) {
+    uint32_t x, y, z;
+    decode(g, n, x, y, z);
+    if (flow_match(flow, n, (int)x, (int)y, (int)z)) {
+#pragma unroll
+      for (int i = 0; i < 27; ++i) f[i] = f[i] + flow.induced[i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 27; ++i) F[pidx<27>(n, i)] = f[i];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_ref_stream_collide(Grid g, const float* __restrict__ A,
+                         float* __restrict__ B, float tau) {
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n) return;
+  uint32_t x, y, z;
+  decode(g, n, x, y, z);
+  float f[27];
+  pull27(g, A, n, (int)x, (int)y, (int)z, f);
+  collide27(f, tau);
+#pragma unroll
+  for (int j = 0; j < 27; ++j) B[pidx<27>(n, j)] = f[j];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_ref_flow(Grid g, float* __restrict__ F, FlowSpec flow) {
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n) return;
+  uint32_t x, y, z;
+  decode(g, n, x, y, z);
+  if (!flow_match(flow, n, (int)x, (int)y, (int)z)) return;
+#pragma unroll
+  for (int i = 0; i < 27; ++i) {
+    float* p = F + pidx<27>(n, i);
+    *p = *p + flow.induced[i];
+  }
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// One thread per node with a nonzero flag byte.  Slot s of a record holds a
+// 5-bit direction d in word s/6, bits 5*(s%6) (Lattice.cpp:450-477); its
+// contribution f_{s/3}(x) * W[s] goes to (x + e_d, d).  The force term uses
+// DIRECTIONS[s]/SPLIT[s] as SimCalcFuncs.cu:181-183 does; for s >= 27 that
+// operand lies outside the reference's tables and is taken as 0 here.
+__global__ void __launch_bounds__(kBlock)
+    k_ref_reflect(Grid g, float* __restrict__ F, int32_t nspecial,
+                  const int32_t* __restrict__ special,
+                  const int32_t* __restrict__ special_rec,
+                  const uint8_t* __restrict__ flags,
+                  const uint32_t* __restrict__ rec_dirs,
+                  const float* __restrict__ rec_w,
+                  float* __restrict__ partials) {
+  const int32_t k = blockIdx.x * kBlock + threadIdx.x;
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  if (k < nspecial) {
+    const uint32_t n = (uint32_t)special[k];
+    const uint8_t fl = flags[n];
+    const int32_t r = special_rec[k];
+    uint32_t x, y, z;
+    decode(g, n, x, y, z);
+    if ((fl & LBM_FLAG_REFLECTED) && r >= 0) {
+      const uint32_t* words = rec_dirs + 14 * (uint64_t)r;
+      const float* W = rec_w + 81 * (uint64_t)r;
+      for (int s = 1; s < 81; ++s) {
+        const int d = (int)((words[s / 6] >> (5 * (s % 6))) & 0x1Fu);
+        if (d == 0 || d > 26) continue;
+        const int tx = (int)x + L27::E[d][0], ty = (int)y + L27::E[d][1],
+                  tz = (int)z + L27::E[d][2];
+        if (tx < 0 || ty < 0 || tz < 0 || tx >= (int)g.nx ||
+            ty >= (int)g.ny || tz >= (int)g.nz)
+          continue;
+        const float val = F[pidx<27>(n, s / 3)] * W[s];
+        float* t = F + pidx<27>((uint32_t)tz + g.nz * ((uint32_t)ty + g.ny * (uint32_t)tx), d);
+        *t = *t + val;
+        const float ts0 = s < 27 ? (float)L27::E[s][0] * kSplit27[s] : 0.0f;
+        const float ts1 = s < 27 ? (float)L27::E[s][1] * kSplit27[s] : 0.0f;
+        const float ts2 = s < 27 ? (float)L27::E[s][2] * kSplit27[s] : 0.0f;
+        fx += val * (((float)L27::E[d][0] * kSplit27[d]) - ts0);
+        fy += val * (((float)L27::E[d][1] * kSplit27[d]) - ts1);
+        fz += val * (((float)L27::E[d][2] * kSplit27[d]) - ts2);
+      }
+    }
+    if (fl & LBM_FLAG_INTERNAL) {
+#pragma unroll
+      for (int i = 0; i < 27; ++i) F[pidx<27>(n, i)] = 0.0f;
+    }
+  }
+  if (partials) {
+    __shared__ float red[3][kBlock / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const float sx = wave_sum(fx), sy = wave_sum(fy), sz = wave_sum(fz);
+    if (lane == 0) {
+      red[0][wv] = sx;
+      red[1][wv] = sy;
+      red[2][wv] = sz;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      float s = 0.0f;
+      for (int w = 0; w < kBlock / 64; ++w) s += red[threadIdx.x][w];
+      partials[3 * blockIdx.x + threadIdx.x] = s;
+    }
+  }
+}
+
+// Adds the per-block partial forces of one reflection pass, summed in FP64
+// in block order, to the FP32 force register (one rounding per pass).
+__global__ void k_ref_force_sum(const float* __restrict__ partials,
+                                int nblocks, float* __restrict__ force) {
+  const int k = threadIdx.x;
+  if (k >= 3) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += (double)partials[3 * b + k];
+  force[k] = (float)((double)force[k] + s);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_ref_moments(Grid g, const float* __restrict__ F, float* __restrict__ rho,
+                  float* __restrict__ u) {
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n) return;
+  float d = 0.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 27; ++i) {
+    const float fi = F[pidx<27>(n, i)];
+    d += fi;
+    ux += fi * (float)L27::E[i][0] * kSplit27[i];
+    uy += fi * (float)L27::E[i][1] * kSplit27[i];
+    uz += fi * (float)L27::E[i][2] * kSplit27[i];
+  }
+  if (d != 0.0f) {
+    ux /= d;
+    uy /= d;
+    uz /= d;
+  }
+  if (rho) rho[n] = d;
+  if (u) {
+    u[n] = ux;
+    u[g.n + n] = uy;
+    u[2 * (uint64_t)g.n + n] = uz;
+  }
+}
+
+inline unsigned blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+int launch_ref_stream(const Grid& g, const float* A, float* B, hipStream_t s) {
+  hipLaunchKernelGGL(k_ref_stream, dim3(blocks_for(g.n)), dim3(kBlock), 0, s,
+                     g, A, B);
+  LBM_HIP(hipGetLastError());
+  return LBM_OK;
+}
+
+int launch_ref_collide(const Grid& g, float* f, float tau,
+                       const FlowSpec* flow, hipStream_t s) {
+  FlowSpec none{};
+  none.kind = LBM_FLOW_NONE;
+  hipLaunchKernelGGL(k_ref_collide, dim3(blocks_for(g.n)), dim3(kBlock), 0, s,
+                     g, f, tau, flow ? *flow : none);
+  LBM_HIP(hipGetLastError());
+  return LBM_OK;
+}
+
+int launch_ref_stream_collide(const Grid& g, const float* A, float* B,
+                              float tau, hipStream_t s) {
+  hipLaunchKernelGGL(k_ref_stream_collide, dim3(blocks_for(g.n)),
+                     dim3(kBlock), 0, s, g, A, B, tau);
+  LBM_HIP(hipGetLastError());
+  return LBM_OK;
+}
+
+int ref_reflect_blocks(int32_t nspecial) {
+  return (int)((nspecial + kBlock - 1) / kBlock);
+}
+
+int launch_ref_reflect(const Grid& g, float* f, int32_t nspecial,
+                       const int32_t* special, const int32_t* special_rec,
+                       const uint8_t* flags, const uint32_t* rec_dirs,
+                       const float* rec_w, float* force_partials,
+                       int nblocks, float* force, hipStream_t s) {
+  if (nspecial <= 0) return LBM_OK;
+  hipLaunchKernelGGL(k_ref_reflect, dim3(nblocks), dim3(kBlock), 0, s, g, f,
+                     nspecial, special, special_rec, flags, rec_dirs, rec_w,
+                     force ? force_partials : nullptr);
+  LBM_HIP(hipGetLastError());
+  if (force) {
+    hipLaunchKernelGGL(k_ref_force_sum, dim3(1), dim3(64), 0, s,
+                       force_partials, nblocks, force);
+    LBM_HIP(hipGetLastError());
+  }
+  return LBM_OK;
+}
+
+int launch_ref_flow(const Grid& g, float* f, const FlowSpec& flow,
+                    hipStream_t s) {
+  if (flow.kind == LBM_FLOW_NONE) return LBM_OK;
+  hipLaunchKernelGGL(k_ref_flow, dim3(blocks_for(g.n)), dim3(kBlock), 0, s, g,
+                     f, flow);
+  LBM_HIP(hipGetLastError());
+  return LBM_OK;
+}
+
+int launch_ref_moments(const Grid& g, const float* f, float* rho, float* u,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_ref_moments, dim3(blocks_for(g.n)), dim3(kBlock), 0, s,
+                     g, f, rho, u);
+  LBM_HIP(hipGetLastError());
+  return LBM_OK;
+}
+
+}  // namespace lbm

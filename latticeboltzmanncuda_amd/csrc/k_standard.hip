@@ -1,0 +1,487 @@
+// Standard-mode kernels: the fused BGK pull collide+stream step, the link
+// mask and node-class builders, equilibrium initialisation, moments and
+// FP64 conservation sums.  Populations use the blocked layout of
+// lbm_layout.h; node index n = z + nz*(y + ny*x).
+//
+// One thread per node over a flat 1-D grid (coalesced along n, whatever the
+// lattice shape); (x, y, z) come from two magic-number divisions.
+//
+// The step, for every node:
+//   1. pulls f_i = A_i(x - e_i) for all i with periodic wrap -- issued
+//      unconditionally, so the loads never wait for the geometry;
+//   2. boundary nodes (2-bit class map, L2-resident) read their link mask
+//      and replace each bounced link by the half-way bounce-back value
+//      A_inv(i)(x) (+ the moving-wall term 6 w_i rho_w e_i.u_w when the
+//      source lies beyond the moving face);
+//   3. collide (BGK + Guo forcing) in FP32 (FP64 for F64 storage);
+//   4. stores every direction with all lanes active; solid nodes store the
+//      rest state w_i.  Skipping the stores of solid lanes leaves partly
+//      written cache lines that cost ~1.7x in time on MI355X.
+// Consecutive steps walk the tiles in opposite orders so that the start of
+// a step reads what the previous step wrote last.
+#include <hip/hip_runtime.h>
+
+#include "lbm_internal.h"
+#include "lbm_lattice.h"
+#include "lbm_layout.h"
+#include "lbm_store.h"
+
+namespace lbm {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ void decode(const Grid& g, uint32_t n, uint32_t& x,
+                                       uint32_t& y, uint32_t& z) {
+  const uint32_t t = g.dz.div(n);
+  z = n - t * g.nz;
+  x = g.dy.div(t);
+  y = t - x * g.ny;
+}
+
+template <int Q, int S, bool LID>
+__global__ void __launch_bounds__(kBlock)
+    k_std_step(Grid g, const typename Store<S>::T* __restrict__ A,
+               typename Store<S>::T* __restrict__ B,
+               const uint32_t* __restrict__ mask,
+               const uint32_t* __restrict__ cls, StdParams p, int rev) {
+  using L = Lat<Q>;
+  using St = Store<S>;
+  using C = typename St::C;
+  const uint32_t blk = rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
+  const uint32_t n = blk * kBlock + threadIdx.x;
+  if (n >= g.n) return;
+  const uint32_t c = node_class(cls, n);
+  uint32_t x, y, z;
+  decode(g, n, x, y, z);
+  const uint32_t plane = g.ny * g.nz;
+  // source coordinate c - e for e = -1, 0, +1 (periodic wrap; a wall link
+  // is flagged in the mask and its wrapped value is replaced)
+  const uint32_t sx[3] = {(x + 1 == g.nx ? 0u : x + 1) * plane, x * plane,
+                          (x == 0 ? g.nx : x) * plane - plane};
+  const uint32_t sy[3] = {(y + 1 == g.ny ? 0u : y + 1) * g.nz, y * g.nz,
+                          (y == 0 ? g.ny : y) * g.nz - g.nz};
+  const uint32_t sz[3] = {z + 1 == g.nz ? 0u : z + 1, z,
+                          (z == 0 ? g.nz : z) - 1};
+
+  C f[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const uint32_t src = sx[L::E[i][0] + 1] + sy[L::E[i][1] + 1] + sz[L::E[i][2] + 1];
+    f[i] = St::load(A + pidx<Q>(src, i));
+  }
+  if (c == kClassBoundary) {
+    const uint32_t m = mask[n];
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      if ((m >> i) & 1u) {
+        C v = St::load(A + pidx<Q>(n, L::INV[i]));
+        if (LID) {
+          const int ex = L::E[i][0], ey = L::E[i][1], ez = L::E[i][2];
+          const int el = p.lid_axis == 0 ? ex : (p.lid_axis == 1 ? ey : ez);
+          if ((m & LBM_MASK_LID) && el == -p.lid_sign) {
+            const C eu = (C)ex * (C)p.lid_u[0] + (C)ey * (C)p.lid_u[1] +
+                         (C)ez * (C)p.lid_u[2];
+            // shifted storage: the wall term is added to f, not to f - w
+            v += (C)6 * (C)L::W[i] * (C)p.wall_rho * eu;
+          }
+        }
+        f[i] = v;
+      }
+    }
+  }
+  if (St::kShifted) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] += (C)L::W[i];
+  }
+
+  C rho = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    rho += f[i];
+    if (L::E[i][0]) jx += L::E[i][0] > 0 ? f[i] : -f[i];
+    if (L::E[i][1]) jy += L::E[i][1] > 0 ? f[i] : -f[i];
+    if (L::E[i][2]) jz += L::E[i][2] > 0 ? f[i] : -f[i];
+  }
+  const C omega = (C)1 / (C)p.tau;
+  const C guo = (C)1 - (C)0.5 * omega;
+  const C Fx = (C)p.force[0], Fy = (C)p.force[1], Fz = (C)p.force[2];
+  const C ir = (C)1 / rho;
+  const C ux = (jx + (C)0.5 * Fx) * ir;
+  const C uy = (jy + (C)0.5 * Fy) * ir;
+  const C uz = (jz + (C)0.5 * Fz) * ir;
+  const C usq = ux * ux + uy * uy + uz * uz;
+  const C uF = ux * Fx + uy * Fy + uz * Fz;
+  const C base = (C)1 - (C)1.5 * usq;
+  const bool solid = c == kClassSolid;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const C ex = (C)L::E[i][0], ey = (C)L::E[i][1], ez = (C)L::E[i][2];
+    const C eu = ex * ux + ey * uy + ez * uz;
+    const C eF = ex * Fx + ey * Fy + ez * Fz;
+    const C w = (C)L::W[i];
+    const C feq = w * rho * (base + (C)3 * eu + (C)4.5 * eu * eu);
+    const C src = guo * w * ((C)3 * (eF - uF) + (C)9 * eu * eF);
+    C out = f[i] + omega * (feq - f[i]) + src;
+    out = solid ? w : out;
+    if (St::kShifted) out -= w;
+    St::store(B + pidx<Q>(n, i), out);
+  }
+}
+
+template <int Q>
+__global__ void __launch_bounds__(kBlock)
+    k_build_mask(Grid g, const uint8_t* __restrict__ solid, int px, int py,
+                 int pz, int lid_face, uint32_t* __restrict__ mask) {
+  using L = Lat<Q>;
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n) return;
+  if (solid && solid[n]) {
+    mask[n] = LBM_MASK_SOLID;
+    return;
+  }
+  uint32_t x, y, z;
+  decode(g, n, x, y, z);
+  const int c[3] = {(int)x, (int)y, (int)z};
+  const int dims[3] = {(int)g.nx, (int)g.ny, (int)g.nz};
+  const int per[3] = {px, py, pz};
+  uint32_t m = 0;
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    int s[3];
+    bool out = false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      s[a] = c[a] - L::E[i][a];
+      if (s[a] < 0 || s[a] >= dims[a]) {
+        if (per[a])
+          s[a] = s[a] < 0 ? s[a] + dims[a] : s[a] - dims[a];
+        else
+          out = true;
+      }
+    }
+    if (!out && solid) {
+      const uint32_t sn = (uint32_t)s[2] + g.nz * ((uint32_t)s[1] + g.ny * (uint32_t)s[0]);
+      out = solid[sn] != 0;
+    }
+    if (out) m |= 1u << i;
+  }
+  if (lid_face >= 0) {
+    const int ax = lid_face / 2, side = lid_face % 2;
+    if ((side == 1 && c[ax] == dims[ax] - 1) || (side == 0 && c[ax] == 0))
+      m |= LBM_MASK_LID;
+  }
+  mask[n] = m;
+}
+
+// Packs the 2-bit node classes of 16 nodes per word from the link masks.
+__global__ void __launch_bounds__(kBlock)
+    k_build_class(Grid g, const uint32_t* __restrict__ mask,
+                  uint32_t* __restrict__ cls) {
+  const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
+  if (w >= (g.n + 15) / 16) return;
+  uint32_t out = 0;
+  for (uint32_t k = 0; k < 16; ++k) {
+    const uint32_t n = w * 16 + k;
+    uint32_t c = kClassInterior;
+    if (n < g.n) {
+      const uint32_t m = mask[n];
+      c = (m & LBM_MASK_SOLID) ? kClassSolid
+                               : (m ? kClassBoundary : kClassInterior);
+    }
+    out |= c << (2 * k);
+  }
+  cls[w] = out;
+}
+
+template <int Q, int S>
+__global__ void __launch_bounds__(kBlock)
+    k_init_eq(Grid g, const double* __restrict__ rho,
+              const double* __restrict__ u, typename Store<S>::T* __restrict__ f) {
+  using L = Lat<Q>;
+  using St = Store<S>;
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n) return;
+  const double r = rho ? rho[n] : 1.0;
+  const double ux = u ? u[n] : 0.0, uy = u ? u[g.n + n] : 0.0,
+               uz = u ? u[2 * (uint64_t)g.n + n] : 0.0;
+  const double usq = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const double eu = L::E[i][0] * ux + L::E[i][1] * uy + L::E[i][2] * uz;
+    double v = L::W[i] * r * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * usq);
+    if (St::kShifted) v -= L::W[i];
+    St::store(f + pidx<Q>(n, i), (typename St::C)v);
+  }
+}
+
+template <int Q, int S>
+__global__ void __launch_bounds__(kBlock)
+    k_std_moments(Grid g, const typename Store<S>::T* __restrict__ f,
+                  const uint32_t* __restrict__ mask, float fx, float fy,
+                  float fz, float* __restrict__ rho, float* __restrict__ u) {
+  using L = Lat<Q>;
+  using St = Store<S>;
+  using C = typename St::C;
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n) return;
+  float r_out = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+  if (!(mask && (mask[n] & LBM_MASK_SOLID))) {
+    C r = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      C v = St::load(f + pidx<Q>(n, i));
+      if (St::kShifted) v += (C)L::W[i];
+      r += v;
+      jx += (C)L::E[i][0] * v;
+      jy += (C)L::E[i][1] * v;
+      jz += (C)L::E[i][2] * v;
+    }
+    r_out = (float)r;
+    ux = (float)((jx - (C)0.5 * (C)fx) / r);
+    uy = (float)((jy - (C)0.5 * (C)fy) / r);
+    uz = (float)((jz - (C)0.5 * (C)fz) / r);
+  }
+  if (rho) rho[n] = r_out;
+  if (u) {
+    u[n] = ux;
+    u[g.n + n] = uy;
+    u[2 * (uint64_t)g.n + n] = uz;
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Per-block FP64 sums of mass and momentum over non-solid nodes; the sum
+// order is fixed (grid-stride per thread, then a fixed tree).
+template <int Q, int S>
+__global__ void __launch_bounds__(kBlock)
+    k_conservation(Grid g, const typename Store<S>::T* __restrict__ f,
+                   const uint32_t* __restrict__ mask,
+                   double* __restrict__ partials) {
+  using L = Lat<Q>;
+  using St = Store<S>;
+  double acc[4] = {0, 0, 0, 0};
+  for (uint32_t n = blockIdx.x * kBlock + threadIdx.x; n < g.n;
+       n += gridDim.x * kBlock) {
+    if (mask && (mask[n] & LBM_MASK_SOLID)) continue;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      double v = (double)St::load(f + pidx<Q>(n, i));
+      if (St::kShifted) v += L::W[i];
+      acc[0] += v;
+      acc[1] += L::E[i][0] * v;
+      acc[2] += L::E[i][1] * v;
+      acc[3] += L::E[i][2] * v;
+    }
+  }
+  __shared__ double red[4][kBlock / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double s = wave_sum(acc[k]);
+    if (lane == 0) red[k][wv] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = 0;
+    for (int w = 0; w < kBlock / 64; ++w) s += red[threadIdx.x][w];
+    partials[4 * blockIdx.x + threadIdx.x] = s;
+  }
+}
+
+__global__ void k_sum_partials(const double* __restrict__ partials,
+                               int nblocks, double* __restrict__ out) {
+  // one wave; lane k < 4 sums component k in block order
+  const int k = threadIdx.x;
+  if (k >= 4) return;
+  double s = 0;
+  for (int b = 0; b < nblocks; ++b) s += partials[4 * b + k];
+  out[k] = s;
+}
+
+template <int Q, int S>
+__global__ void __launch_bounds__(kBlock)
+    k_convert_in(Grid g, const void* __restrict__ src, int is_f64,
+                 typename Store<S>::T* __restrict__ f) {
+  using L = Lat<Q>;
+  using St = Store<S>;
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n) return;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const uint64_t k = (uint64_t)i * g.n + n;
+    double v = is_f64 ? ((const double*)src)[k] : (double)((const float*)src)[k];
+    if (St::kShifted) {
+      // f32 inputs are shifted in f32 so that f32 -> store -> f32 is exact
+      // wherever the storage can hold the deviation exactly
+      if (is_f64)
+        v -= L::W[i];
+      else
+        v = (double)(((const float*)src)[k] - (float)L::W[i]);
+    }
+    St::store(f + pidx<Q>(n, i), (typename St::C)v);
+  }
+}
+
+template <int Q, int S>
+__global__ void __launch_bounds__(kBlock)
+    k_convert_out(Grid g, const typename Store<S>::T* __restrict__ f,
+                  void* __restrict__ dst, int is_f64) {
+  using L = Lat<Q>;
+  using St = Store<S>;
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n) return;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const uint64_t k = (uint64_t)i * g.n + n;
+    const typename St::C v = St::load(f + pidx<Q>(n, i));
+    if (is_f64) {
+      double d = (double)v;
+      if (St::kShifted) d += L::W[i];
+      ((double*)dst)[k] = d;
+    } else {
+      float d = (float)v;
+      if (St::kShifted) d += (float)L::W[i];
+      ((float*)dst)[k] = d;
+    }
+  }
+}
+
+inline unsigned blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+
+}  // namespace
+
+int launch_std_step(int q, int storage, const Grid& g, const void* A, void* B,
+                    const uint32_t* mask, const uint32_t* cls,
+                    const StdParams& p, int rev, hipStream_t s) {
+  const unsigned nb = blocks_for(g.n);
+  const bool lid = p.lid_axis >= 0;
+  const int drc = dispatch_qs(q, storage, [&](auto tag) {
+    using Tag = decltype(tag);
+    using T = typename Store<Tag::S>::T;
+    if (lid)
+      hipLaunchKernelGGL((k_std_step<Tag::Q, Tag::S, true>), dim3(nb),
+                         dim3(kBlock), 0, s, g, (const T*)A, (T*)B, mask, cls,
+                         p, rev);
+    else
+      hipLaunchKernelGGL((k_std_step<Tag::Q, Tag::S, false>), dim3(nb),
+                         dim3(kBlock), 0, s, g, (const T*)A, (T*)B, mask, cls,
+                         p, rev);
+  });
+  if (drc) return drc;
+  LBM_HIP(hipGetLastError());
+  return LBM_OK;
+}
+
+int launch_build_mask(int q, const Grid& g, const uint8_t* solid,
+                      const int periodic[3], int lid_face, uint32_t* mask,
+                      uint32_t* cls, hipStream_t s) {
+  const unsigned nb = blocks_for(g.n);
+  switch (q) {
+    case 9:
+      hipLaunchKernelGGL(k_build_mask<9>, dim3(nb), dim3(kBlock), 0, s, g,
+                         solid, periodic[0], periodic[1], periodic[2],
+                         lid_face, mask);
+      break;
+    case 19:
+      hipLaunchKernelGGL(k_build_mask<19>, dim3(nb), dim3(kBlock), 0, s, g,
+                         solid, periodic[0], periodic[1], periodic[2],
+                         lid_face, mask);
+      break;
+    case 27:
+      hipLaunchKernelGGL(k_build_mask<27>, dim3(nb), dim3(kBlock), 0, s, g,
+                         solid, periodic[0], periodic[1], periodic[2],
+                         lid_face, mask);
+      break;
+    default:
+      return fail(LBM_ERR_UNSUPPORTED, "lattice not built");
+  }
+  LBM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_build_class, dim3(blocks_for((g.n + 15) / 16)),
+                     dim3(kBlock), 0, s, g, mask, cls);
+  LBM_HIP(hipGetLastError());
+  return LBM_OK;
+}
+
+int launch_init_eq(int q, int storage, const Grid& g, const double* rho,
+                   const double* u, void* f, hipStream_t s) {
+  const unsigned nb = blocks_for(g.n);
+  const int drc = dispatch_qs(q, storage, [&](auto tag) {
+    using Tag = decltype(tag);
+    using T = typename Store<Tag::S>::T;
+    hipLaunchKernelGGL((k_init_eq<Tag::Q, Tag::S>), dim3(nb), dim3(kBlock), 0, s, g,
+                       rho, u, (T*)f);
+  });
+  if (drc) return drc;
+  LBM_HIP(hipGetLastError());
+  return LBM_OK;
+}
+
+int launch_std_moments(int q, int storage, const Grid& g, const void* f,
+                       const uint32_t* mask, const float force[3], float* rho,
+                       float* u, hipStream_t s) {
+  const unsigned nb = blocks_for(g.n);
+  const int drc = dispatch_qs(q, storage, [&](auto tag) {
+    using Tag = decltype(tag);
+    using T = typename Store<Tag::S>::T;
+    hipLaunchKernelGGL((k_std_moments<Tag::Q, Tag::S>), dim3(nb), dim3(kBlock), 0, s,
+                       g, (const T*)f, mask, force[0], force[1], force[2],
+                       rho, u);
+  });
+  if (drc) return drc;
+  LBM_HIP(hipGetLastError());
+  return LBM_OK;
+}
+
+int launch_conservation(int q, int storage, const Grid& g, const void* f,
+                        const uint32_t* mask, double* partials, int nblocks,
+                        double* out4, hipStream_t s) {
+  const int drc = dispatch_qs(q, storage, [&](auto tag) {
+    using Tag = decltype(tag);
+    using T = typename Store<Tag::S>::T;
+    hipLaunchKernelGGL((k_conservation<Tag::Q, Tag::S>), dim3(nblocks), dim3(kBlock),
+                       0, s, g, (const T*)f, mask, partials);
+  });
+  if (drc) return drc;
+  LBM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, partials,
+                     nblocks, out4);
+  LBM_HIP(hipGetLastError());
+  return LBM_OK;
+}
+
+int launch_convert_in(int q, int storage, const Grid& g, const void* src,
+                      int src_is_f64, void* f, hipStream_t s) {
+  const unsigned nb = blocks_for(g.n);
+  const int drc = dispatch_qs(q, storage, [&](auto tag) {
+    using Tag = decltype(tag);
+    using T = typename Store<Tag::S>::T;
+    hipLaunchKernelGGL((k_convert_in<Tag::Q, Tag::S>), dim3(nb), dim3(kBlock), 0, s,
+                       g, src, src_is_f64, (T*)f);
+  });
+  if (drc) return drc;
+  LBM_HIP(hipGetLastError());
+  return LBM_OK;
+}
+
+int launch_convert_out(int q, int storage, const Grid& g, const void* f,
+                       void* dst, int dst_is_f64, hipStream_t s) {
+  const unsigned nb = blocks_for(g.n);
+  const int drc = dispatch_qs(q, storage, [&](auto tag) {
+    using Tag = decltype(tag);
+    using T = typename Store<Tag::S>::T;
+    hipLaunchKernelGGL((k_convert_out<Tag::Q, Tag::S>), dim3(nb), dim3(kBlock), 0, s,
+                       g, (const T*)f, dst, dst_is_f64);
+  });
+  if (drc) return drc;
+  LBM_HIP(hipGetLastError());
+  return LBM_OK;
+}
+
+}  // namespace lbm

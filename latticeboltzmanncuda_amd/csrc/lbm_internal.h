@@ -1,0 +1,165 @@
+// Internal state of a lattice context and the launchers each kernel file
+// exports to the C-ABI layer.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm.h"
+
+namespace lbm {
+
+// Division by a runtime constant d for dividends below 2^31:
+// q = (umulhi(n, mul) + n) >> shift  (Granlund-Montgomery round-up method).
+struct Magic {
+  uint32_t mul;
+  uint32_t shift;
+  uint32_t d;
+  __host__ __device__ uint32_t div(uint32_t n) const {
+#ifdef __HIP_DEVICE_COMPILE__
+    const uint32_t t = __umulhi(n, mul);
+#else
+    const uint32_t t = (uint32_t)(((uint64_t)n * mul) >> 32);
+#endif
+    return (t + n) >> shift;
+  }
+};
+
+inline Magic make_magic(uint32_t d) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  const uint64_t m = ((1ull << 32) * ((1ull << l) - d)) / d + 1;
+  return Magic{(uint32_t)m, l, d};
+}
+
+// Geometry of a lattice as the kernels see it.
+struct Grid {
+  uint32_t nx, ny, nz;
+  uint32_t n;    // nodes
+  Magic dz, dy;  // division by nz and by ny
+};
+
+// Parameters of the standard (BGK) step.
+struct StdParams {
+  float tau;       // relaxation time
+  float force[3];  // Guo body force
+  int lid_axis;    // -1: no moving wall
+  int lid_sign;    // +1 max face, -1 min face
+  float lid_u[3];
+  float wall_rho;
+};
+
+struct FlowSpec {
+  int kind;
+  int32_t params[6];
+  float induced[27];
+  const uint8_t* mask;
+};
+
+// error helpers --------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define LBM_HIP(call)                                     \
+  do {                                                    \
+    hipError_t e_ = (call);                               \
+    if (e_ != hipSuccess) return ::lbm::hip_fail(e_, #call); \
+  } while (0)
+
+// launchers: standard mode (k_standard.hip) -----------------------------
+int launch_std_step(int q, int storage, const Grid& g, const void* A, void* B,
+                    const uint32_t* mask, const uint32_t* cls,
+                    const StdParams& p, int rev, hipStream_t s);
+int launch_build_mask(int q, const Grid& g, const uint8_t* solid,
+                      const int periodic[3], int lid_face, uint32_t* mask,
+                      uint32_t* cls, hipStream_t s);
+int launch_init_eq(int q, int storage, const Grid& g, const double* rho,
+                   const double* u, void* f, hipStream_t s);
+int launch_std_moments(int q, int storage, const Grid& g, const void* f,
+                       const uint32_t* mask, const float force[3], float* rho,
+                       float* u, hipStream_t s);
+int launch_conservation(int q, int storage, const Grid& g, const void* f,
+                        const uint32_t* mask, double* partials, int nblocks,
+                        double* out4, hipStream_t s);
+int launch_convert_in(int q, int storage, const Grid& g, const void* src,
+                      int src_is_f64, void* f, hipStream_t s);
+int launch_convert_out(int q, int storage, const Grid& g, const void* f,
+                       void* dst, int dst_is_f64, hipStream_t s);
+
+// launchers: reference mode (k_reference.hip) ---------------------------
+int launch_ref_stream(const Grid& g, const float* A, float* B, hipStream_t s);
+int launch_ref_collide(const Grid& g, float* f, float tau,
+                       const FlowSpec* flow, hipStream_t s);
+int launch_ref_stream_collide(const Grid& g, const float* A, float* B,
+                              float tau, hipStream_t s);
+int launch_ref_reflect(const Grid& g, float* f, int32_t nspecial,
+                       const int32_t* special, const int32_t* special_rec,
+                       const uint8_t* flags, const uint32_t* rec_dirs,
+                       const float* rec_w, float* force_partials,
+                       int nblocks, float* force, hipStream_t s);
+int ref_reflect_blocks(int32_t nspecial);
+int launch_ref_flow(const Grid& g, float* f, const FlowSpec& flow,
+                    hipStream_t s);
+int launch_ref_moments(const Grid& g, const float* f, float* rho, float* u,
+                       hipStream_t s);
+
+// launchers: shared (k_common.hip) --------------------------------------
+int launch_points_in(int q, int storage, const Grid& g,
+                     const lbm_lattice_point* pts, void* f, uint8_t* flags,
+                     hipStream_t s);
+int launch_points_out(int q, int storage, const Grid& g, const void* f,
+                      const uint8_t* flags, lbm_lattice_point* pts,
+                      hipStream_t s);
+int launch_test_coordinates(const Grid& g, uint32_t* counts, int* ok,
+                            hipStream_t s);
+
+}  // namespace lbm
+
+// The context behind the opaque lbm_ctx handle.
+struct lbm_ctx {
+  lbm_config cfg{};
+  int q = 0;
+  lbm::Grid grid{};
+  size_t elem = 4;
+  void* f[2] = {nullptr, nullptr};
+  int cur = 0;
+  uint32_t* mask = nullptr;  // standard mode: link mask per node
+  uint32_t* cls = nullptr;   // standard mode: 2-bit node classes
+  uint64_t steps_done = 0;   // parity picks the tile order of the next step
+  uint8_t* flags = nullptr;  // reference mode
+  std::vector<uint8_t> host_flags;
+
+  // reflection records (reference mode)
+  int32_t nrec = 0;
+  int32_t* rec_node = nullptr;
+  uint32_t* rec_dirs = nullptr;
+  float* rec_w = nullptr;
+  std::vector<int32_t> host_rec_node;
+  // nodes with a nonzero flag byte, and their record (or -1)
+  int32_t nspecial = 0;
+  int32_t* special = nullptr;
+  int32_t* special_rec = nullptr;
+  bool special_dirty = true;
+
+  // flow injection
+  lbm::FlowSpec flow{};
+  uint8_t* flow_mask = nullptr;
+
+  // force register and reduction scratch
+  float* force = nullptr;
+  float* force_partials = nullptr;
+  int force_partials_cap = 0;
+  double* red = nullptr;
+  int red_cap = 0;
+  double* red_out = nullptr;
+
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int device = 0;
+  int64_t device_bytes = 0;
+};

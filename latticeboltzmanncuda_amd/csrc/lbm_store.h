@@ -1,0 +1,103 @@
+// Population storage types.  F32/F64 store f_i itself; F16/BF16 store the
+// shifted deviation f_i - w_i (the rest-state value w_i carries most of the
+// magnitude, so the 16-bit mantissa spends its bits on the deviation).
+// Conversions to 16 bits round to nearest, ties to even.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/lbm.h"
+#include "lbm_internal.h"
+
+namespace lbm {
+
+template <int S>
+struct Store;
+
+template <>
+struct Store<LBM_STORE_F32> {
+  using T = float;  // stored
+  using C = float;  // computed
+  static constexpr bool kShifted = false;
+  __device__ static C load(const T* p) { return *p; }
+  __device__ static void store(T* p, C v) { *p = v; }
+};
+
+template <>
+struct Store<LBM_STORE_F64> {
+  using T = double;
+  using C = double;
+  static constexpr bool kShifted = false;
+  __device__ static C load(const T* p) { return *p; }
+  __device__ static void store(T* p, C v) { *p = v; }
+};
+
+template <>
+struct Store<LBM_STORE_F16> {
+  using T = _Float16;
+  using C = float;
+  static constexpr bool kShifted = true;
+  __device__ static C load(const T* p) { return (float)*p; }
+  __device__ static void store(T* p, C v) { *p = (_Float16)v; }
+};
+
+template <>
+struct Store<LBM_STORE_BF16> {
+  using T = uint16_t;
+  using C = float;
+  static constexpr bool kShifted = true;
+  __device__ static C load(const T* p) {
+    return __uint_as_float(((uint32_t)*p) << 16);
+  }
+  __device__ static void store(T* p, C v) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    *p = (uint16_t)(u >> 16);
+  }
+};
+
+inline size_t storage_bytes(int storage) {
+  switch (storage) {
+    case LBM_STORE_F32:
+      return 4;
+    case LBM_STORE_F64:
+      return 8;
+    case LBM_STORE_F16:
+    case LBM_STORE_BF16:
+      return 2;
+    default:
+      return 0;
+  }
+}
+
+// Calls fn(QS<Q, S>{}) for the runtime pair (q, storage), so that a
+// generic lambda can instantiate kernels for compile-time Q and storage.
+template <int Qv, int Sv>
+struct QS {
+  static constexpr int Q = Qv;
+  static constexpr int S = Sv;
+};
+
+template <class Fn>
+int dispatch_qs(int q, int storage, Fn&& fn) {
+  switch (q * 8 + storage) {
+    case 9 * 8 + LBM_STORE_F32: fn(QS<9, LBM_STORE_F32>{}); return LBM_OK;
+    case 9 * 8 + LBM_STORE_F64: fn(QS<9, LBM_STORE_F64>{}); return LBM_OK;
+    case 9 * 8 + LBM_STORE_F16: fn(QS<9, LBM_STORE_F16>{}); return LBM_OK;
+    case 9 * 8 + LBM_STORE_BF16: fn(QS<9, LBM_STORE_BF16>{}); return LBM_OK;
+    case 19 * 8 + LBM_STORE_F32: fn(QS<19, LBM_STORE_F32>{}); return LBM_OK;
+    case 19 * 8 + LBM_STORE_F64: fn(QS<19, LBM_STORE_F64>{}); return LBM_OK;
+    case 19 * 8 + LBM_STORE_F16: fn(QS<19, LBM_STORE_F16>{}); return LBM_OK;
+    case 19 * 8 + LBM_STORE_BF16: fn(QS<19, LBM_STORE_BF16>{}); return LBM_OK;
+    case 27 * 8 + LBM_STORE_F32: fn(QS<27, LBM_STORE_F32>{}); return LBM_OK;
+    case 27 * 8 + LBM_STORE_F64: fn(QS<27, LBM_STORE_F64>{}); return LBM_OK;
+    case 27 * 8 + LBM_STORE_F16: fn(QS<27, LBM_STORE_F16>{}); return LBM_OK;
+    case 27 * 8 + LBM_STORE_BF16: fn(QS<27, LBM_STORE_BF16>{}); return LBM_OK;
+    default:
+      return fail(LBM_ERR_UNSUPPORTED, "lattice/storage combination not built");
+  }
+}
+
+}  // namespace lbm

@@ -153,11 +153,15 @@ static inline void set_slot_dir(uint32_t* words, int s, unsigned v) {
  * The reference indexes DIRECTIONS/SPLIT with the slot s (not s/3); for
  * s >= 27 that read is past the table and is defined here as 0. The
  * reference adds with atomics in an unspecified order; nodes are visited
- * here in index order and slots in ascending order. */
+ * here in index order and slots in ascending order.  force (nullable)
+ * receives every term in float, one addition per term, as the reference's
+ * atomics do; force64 (nullable) receives the exact (FP64) sum of this
+ * pass. */
 void orc_ref_reflect(int nx, int ny, int nz, const float* L, float* T,
                      const uint8_t* flags, int nrec, const int32_t* rec_node,
                      const uint32_t* rec_dirs, const float* rec_w,
-                     float* force) {
+                     float* force, double* force64) {
+  double acc[3] = {0.0, 0.0, 0.0};
   const int64_t N = (int64_t)nx * ny * nz;
   memcpy(T, L, sizeof(float) * 27 * (size_t)N);
   int32_t* rec_of = (int32_t*)malloc(sizeof(int32_t) * (size_t)N);
@@ -180,13 +184,13 @@ void orc_ref_reflect(int nx, int ny, int nz, const float* L, float* T,
             if (!in_box(tx, ty, tz, nx, ny, nz)) continue;
             const float val = L[(s / 3) * N + n] * W[s];
             T[d * N + idx3(tx, ty, tz, ny, nz)] += val;
-            if (force) {
-              for (int a = 0; a < 3; ++a) {
-                const float td = (float)ORC_E27[d][a] * ORC_SPLIT27[d];
-                const float ts =
-                    s < 27 ? (float)ORC_E27[s][a] * ORC_SPLIT27[s] : 0.0f;
-                force[a] += val * (td - ts);
-              }
+            for (int a = 0; a < 3; ++a) {
+              const float td = (float)ORC_E27[d][a] * ORC_SPLIT27[d];
+              const float ts =
+                  s < 27 ? (float)ORC_E27[s][a] * ORC_SPLIT27[s] : 0.0f;
+              const float term = val * (td - ts);
+              if (force) force[a] += term;
+              acc[a] += (double)term;
             }
           }
         }
@@ -194,6 +198,8 @@ void orc_ref_reflect(int nx, int ny, int nz, const float* L, float* T,
           for (int i = 0; i < 27; ++i) T[i * N + n] = 0.0f;
       }
   free(rec_of);
+  if (force64)
+    for (int a = 0; a < 3; ++a) force64[a] += acc[a];
 }
 
 static inline int flow_match(int kind, const int32_t* p, int x, int y, int z) {
@@ -231,13 +237,14 @@ void orc_ref_run_main(int nx, int ny, int nz, float* f, const uint8_t* flags,
                       int nrec, const int32_t* rec_node,
                       const uint32_t* rec_dirs, const float* rec_w, float tau,
                       int flow_kind, const int32_t* flow_params,
-                      const float* induced, float* force, int steps) {
+                      const float* induced, float* force, double* force64,
+                      int steps) {
   const int64_t N = (int64_t)nx * ny * nz;
   float* T = (float*)malloc(sizeof(float) * 27 * (size_t)N);
   for (int s = 0; s < steps; ++s) {
     orc_ref_stream(nx, ny, nz, f, T);
     orc_ref_reflect(nx, ny, nz, T, f, flags, nrec, rec_node, rec_dirs, rec_w,
-                    force);
+                    force, force64);
     orc_ref_collide(nx, ny, nz, f, tau);
     orc_ref_flow(nx, ny, nz, f, flow_kind, flow_params, induced);
   }
@@ -858,7 +865,10 @@ float orc_round_bf16(float x) {
         for (int z = 0; z < nz; ++z) {                                         \
           const int64_t n = idx3(x, y, z, ny, nz);                             \
           const uint32_t m = mask[n];                                          \
-          if (m & ORC_MASK_SOLID) continue;                                    \
+          if (m & ORC_MASK_SOLID) {                                            \
+            for (int i = 0; i < q; ++i) STORE(B, i * N + n, i, wt[i]);         \
+            continue;                                                          \
+          }                                                                    \
           const int c[3] = {x, y, z};                                          \
           REAL f[27];                                                          \
           for (int i = 0; i < q; ++i) {                                        \

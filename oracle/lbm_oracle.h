@@ -19,6 +19,9 @@
  *                           simulation/source/FlowCriterion.cu:5-16
  *       orc_ref_run_main    LatticeBoltzmannCuda.cpp:161-169 (S,R,C,F order)
  *       orc_ref_run_lattice simulation/source/Lattice.cpp:222-226 (S then C)
+ *   The force accumulator is kept twice: in float with one addition per
+ *   term (the reference's atomicAdd semantics, SimCalcFuncs.cu:181-183) and
+ *   in FP64 per pass (the exact sum, which the GPU register tracks).
  *
  *  2. Geometry (host setup of the reference):
  *       orc_mesh_*              simulation/source/Model.cpp:22-91 (CGAL
@@ -93,7 +96,8 @@ void orc_ref_collide(int nx, int ny, int nz, float* f, float tau);
 void orc_ref_reflect(int nx, int ny, int nz, const float* L, float* T,
                      const uint8_t* flags, int nrec, const int32_t* rec_node,
                      const uint32_t* rec_dirs, const float* rec_w,
-                     float* force /* nullable, 3 floats, accumulated */);
+                     float* force /* nullable, 3 floats, accumulated */,
+                     double* force64 /* nullable, FP64 sum of the pass */);
 void orc_ref_flow(int nx, int ny, int nz, float* f, int kind,
                   const int32_t* params, const float* induced);
 /* Whole time loop of LatticeBoltzmannCuda.cpp:161-169: per step stream,
@@ -102,7 +106,8 @@ void orc_ref_run_main(int nx, int ny, int nz, float* f, const uint8_t* flags,
                       int nrec, const int32_t* rec_node,
                       const uint32_t* rec_dirs, const float* rec_w, float tau,
                       int flow_kind, const int32_t* flow_params,
-                      const float* induced, float* force, int steps);
+                      const float* induced, float* force, double* force64,
+                      int steps);
 /* Lattice::simulateLattice repeated: stream then collide. */
 void orc_ref_run_lattice(int nx, int ny, int nz, float* f, float tau,
                          int steps);
@@ -162,7 +167,8 @@ typedef struct orc_std_params {
 void orc_std_init_eq_f64(int q, int nx, int ny, int nz, const double* rho,
                          const double* u, double* f);
 /* storage: 0 float, 2 fp16 shifted, 3 bf16 shifted (values f_i - w_i are
- * rounded to the storage type after every step).  fA is in/out (float). */
+ * rounded to the storage type after every step).  fA is in/out (float).
+ * Solid nodes are set to the rest state w_i by every step. */
 void orc_std_run_f32(int q, int nx, int ny, int nz, float* fA,
                      const uint32_t* mask, const orc_std_params* p, int steps,
                      int storage);

@@ -71,11 +71,12 @@ def _declare(L: C.CDLL) -> None:
     L.orc_get_threads.restype = C.c_int
     L.orc_ref_stream.argtypes = [C.c_int, C.c_int, C.c_int, P, P]
     L.orc_ref_collide.argtypes = [C.c_int, C.c_int, C.c_int, P, C.c_float]
-    L.orc_ref_reflect.argtypes = [C.c_int] * 3 + [P, P, P, C.c_int, P, P, P, P]
+    L.orc_ref_reflect.argtypes = [C.c_int] * 3 + [P, P, P, C.c_int, P, P, P, P,
+                                                  P]
     L.orc_ref_flow.argtypes = [C.c_int] * 3 + [P, C.c_int, P, P]
     L.orc_ref_run_main.argtypes = [C.c_int] * 3 + [P, P, C.c_int, P, P, P,
                                                   C.c_float, C.c_int, P, P, P,
-                                                  C.c_int]
+                                                  P, C.c_int]
     L.orc_ref_run_lattice.argtypes = [C.c_int] * 3 + [P, C.c_float, C.c_int]
     L.orc_ref_moments.argtypes = [C.c_int] * 3 + [P, P, P]
     L.orc_mesh_load.argtypes = [C.c_char_p]
@@ -139,11 +140,14 @@ def ref_collide(dims, f: np.ndarray, tau: float) -> np.ndarray:
     return out
 
 
-def ref_reflect(dims, f, flags, records, force=None):
+def ref_reflect(dims, f, flags, records, force=None, force64=None):
+    """force: float accumulator, one addition per term (reference
+    semantics); force64: FP64 sum of the pass added in."""
     out = np.empty_like(f)
     node, dirs, w = records
     lib().orc_ref_reflect(*dims, _ptr(f), _ptr(out), _ptr(flags), len(node),
-                          _ptr(node), _ptr(dirs), _ptr(w), _ptr(force))
+                          _ptr(node), _ptr(dirs), _ptr(w), _ptr(force),
+                          _ptr(force64))
     return out
 
 
@@ -156,14 +160,15 @@ def ref_flow(dims, f, kind, params, induced):
 
 
 def ref_run_main(dims, f, flags, records, tau, flow_kind, flow_params, induced,
-                 force, steps):
+                 force, steps, force64=None):
     out = f.copy()
     node, dirs, w = records
     flow_params = np.ascontiguousarray(flow_params, dtype=np.int32)
     induced = np.ascontiguousarray(induced, dtype=np.float32)
     lib().orc_ref_run_main(*dims, _ptr(out), _ptr(flags), len(node), _ptr(node),
                            _ptr(dirs), _ptr(w), tau, flow_kind,
-                           _ptr(flow_params), _ptr(induced), _ptr(force), steps)
+                           _ptr(flow_params), _ptr(induced), _ptr(force),
+                           _ptr(force64), steps)
     return out
 
 

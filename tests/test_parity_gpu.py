@@ -1,0 +1,454 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle and the
+reference's golden values.  Every test here needs an MI355X."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from latticeboltzmanncuda_amd import capi
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+E27, INV27, _ = O.std_tables(27)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def idx(x, y, z, dims):
+    return z + dims[2] * (y + dims[1] * x)
+
+
+def ref_ctx(dims, tau=1.0):
+    return capi.Context(capi.make_config(*dims, lattice=27,
+                                         mode=capi.MODE_REFERENCE, tau=tau))
+
+
+def rel_err(a, b):
+    """max |a - b| / max |b| over the whole field."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+# Velocity is compared as one vector field, relative to max(|u|, U_FLOOR):
+# u = j/rho is a small difference of O(0.1) populations, so two FP32
+# evaluations that differ only in rounding order disagree in absolute terms
+# by ~the population error (~1e-7 after a few hundred steps) whatever the
+# flow speed.  U_FLOOR = 0.1 lattice units (Mach ~0.17) is the velocity
+# scale of the benchmark flows.
+U_FLOOR = 0.1
+TOL = 1e-5
+
+
+def vel_err(ug, uo):
+    ug = np.asarray(ug, np.float64)
+    uo = np.asarray(uo, np.float64)
+    return np.abs(ug - uo).max() / max(np.abs(uo).max(), U_FLOOR)
+
+
+# ------------------------------------------------ reference golden values
+def test_streaming_golden():
+    # CalculationsTest.cpp:19-45
+    say("case: streaming golden 1x2x1")
+    with ref_ctx((1, 2, 1)) as c:
+        pts = np.zeros(2, capi.POINT_DTYPE)
+        pts["f"][0, 5] = 2.0
+        c.upload_points(pts)
+        c.stream()
+        out = c.download_points()
+        assert abs(out["f"][1, 5] - 2.0) < 1e-6
+        assert out["f"][0, 5] == 0.0
+        assert list(out["y"]) == [0, 1]
+
+
+def test_collision_golden():
+    # CalculationsTest.cpp:47-75
+    say("case: collision golden 1x1x1")
+    with ref_ctx((1, 1, 1)) as c:
+        pts = np.zeros(1, capi.POINT_DTYPE)
+        pts["f"][0, 0] = 2.0
+        c.upload_points(pts)
+        c.collide(1.0)
+        f = c.download_points()["f"][0]
+        assert abs(f[0] - 0.592592593) < 1e-6
+        assert abs(f[1] - 0.148148148) < 1e-6
+        assert abs(f[2] - 0.148148148) < 1e-6
+
+
+def test_full_golden():
+    # CalculationsTest.cpp:77-105 (Lattice::simulateLattice = fused step)
+    say("case: full golden 1x2x1")
+    for fused in (True, False):
+        with ref_ctx((1, 2, 1), tau=1.0) as c:
+            pts = np.zeros(2, capi.POINT_DTYPE)
+            pts["f"][0, 5] = 2.0
+            c.upload_points(pts)
+            if fused:
+                c.step(1)
+            else:
+                c.stream()
+                c.collide(1.0)
+            f = c.download_points()["f"][1]
+            assert abs(f[0] - -0.296296296) < 1e-6
+            assert abs(f[1] - -0.074074074) < 1e-6
+            assert abs(f[2] - 0.148148148) < 1e-6
+
+
+def _angled(data_dir, dims=(3, 3, 3), spacing=0.5):
+    mesh = O.Mesh(os.path.join(data_dir, "cube_angled.obj"))
+    ok, internal = mesh.insert(dims, spacing)
+    assert ok
+    flags, recs = mesh.preprocess(dims, spacing, internal * O.FLAG_INTERNAL)
+    return flags, recs
+
+
+def test_reflections_golden(data_dir):
+    # CalculationsTest.cpp:107-197
+    say("case: reflections golden 3x3x3 cube_angled")
+    dims = (3, 3, 3)
+    flags, (node, dirs, w) = _angled(data_dir)
+    with ref_ctx(dims) as c:
+        c.upload_flags(flags)
+        c.upload_reflections(node, dirs, w)
+        pts = c.download_points()
+        assert pts["isInternal"][13] == 1 and pts["isReflected"][13] == 1
+        pts["f"][idx(1, 1, 1, dims), 1] = 1.0
+        c.upload_points(pts)
+        c.set_force([0, 0, 0])
+        c.reflect(with_force=True)
+        out = c.download_points()["f"]
+        assert abs(out[idx(2, 1, 2, dims), 17] - 0.367402) < 1e-6
+        assert abs(out[idx(1, 1, 2, dims), 4] - 0.341121) < 1e-6
+        assert abs(out[idx(2, 2, 2, dims), 26] - 0.291477) < 1e-6
+        assert abs(out[idx(1, 1, 1, dims), 1]) < 1e-6
+        F = c.force()
+        assert abs(F[0] - 0.28086) < 1e-6
+        assert abs(F[1] - -0.194318) < 1e-6
+        assert abs(F[2] - 0.648261) < 1e-6
+
+
+def test_flow_golden():
+    # CalculationsTest.cpp:199-277
+    say("case: flow golden 3x3x3")
+    dims = (3, 3, 3)
+    induced = np.zeros(27, np.float32)
+    induced[0] = 1.0
+    with ref_ctx(dims) as c:
+        c.set_flow(capi.FLOW_POINT, [1, 1, 1], induced)
+        c.flow()
+        f = c.download_points()["f"]
+        assert abs(f[idx(1, 1, 1, dims), 0] - 1.0) < 1e-6
+        assert abs(f[idx(0, 0, 0, dims), 0]) < 1e-6
+    with ref_ctx(dims) as c:
+        c.set_flow(capi.FLOW_PLANE, [0, 0], induced)
+        c.flow()
+        f = c.download_points()["f"]
+        for y in range(3):
+            for z in range(3):
+                assert abs(f[idx(0, y, z, dims), 0] - 1.0) < 1e-6
+                assert abs(f[idx(1, y, z, dims), 0]) < 1e-6
+
+
+def test_coordinates_100_cubed():
+    # CoordinatesTest.cpp:14-135: load/retrieve round trip, index
+    # z + 100y + 10000x, device-side mapping check.
+    say("case: coordinates 100^3")
+    dims = (100, 100, 100)
+    with ref_ctx(dims) as c:
+        pts = np.zeros(10 ** 6, capi.POINT_DTYPE)
+        g = np.indices(dims).reshape(3, -1)
+        pts["x"], pts["y"], pts["z"] = g[0], g[1], g[2]
+        rng = np.random.default_rng(0)
+        pts["f"] = rng.random((10 ** 6, 27), dtype=np.float32)
+        c.upload_points(pts)
+        out = c.download_points()
+        assert (out["x"] == g[0]).all() and (out["y"] == g[1]).all()
+        assert (out["z"] == g[2]).all()
+        n = 7 + 100 * 3 + 10000 * 42
+        assert (out["x"][n], out["y"][n], out["z"][n]) == (42, 3, 7)
+        assert np.array_equal(out["f"], pts["f"])
+        assert c.test_coordinates()
+
+
+# ------------------------------------------------ reference mode vs oracle
+def test_ref_stages_bit_exact_vs_oracle():
+    say("case: reference stages vs oracle 7x9x11")
+    dims = (7, 9, 11)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(1)
+    f = (0.001 + 0.001 * rng.random((27, n))).astype(np.float32)
+    with ref_ctx(dims, tau=0.6) as c:
+        c.set_populations(f.reshape(-1))
+        c.stream()
+        g = c.populations()
+        assert np.array_equal(g, O.ref_stream(dims, f.reshape(-1)))
+        c.collide(0.6)
+        h = c.populations()
+        o = O.ref_collide(dims, g, 0.6)
+        # same operation order, no contraction: expect identical floats
+        assert rel_err(h, o) < 1e-6
+        say(f"  collide max ulp-level diff {np.abs(h - o).max():.3e}")
+
+
+def test_p2_reference_lattice_100_steps():
+    # simulateLattice (stream then collide), random f around 0.001, seed 1
+    say("case: P2 reference S->C 32^3 x 100 steps")
+    dims = (32, 32, 32)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(1)
+    f = (0.001 * (1 + 0.1 * rng.standard_normal(27 * n))).astype(np.float32)
+    for tau in (0.6, 1.0):
+        with ref_ctx(dims, tau=tau) as c:
+            c.set_populations(f)
+            c.step(100)
+            g = c.populations()
+            rho_g, u_g = c.moments()
+        o = O.ref_run_lattice(dims, f, tau, 100)
+        rho_o, u_o = O.ref_moments(dims, o)
+        assert rel_err(g, o) <= TOL
+        assert rel_err(rho_g, rho_o) <= TOL
+        assert vel_err(u_g, u_o) <= TOL
+
+
+def _p1_setup(data_dir, dims, spacing, mesh_name):
+    mesh = O.Mesh(os.path.join(data_dir, mesh_name))
+    ok, internal = mesh.insert(dims, spacing)
+    assert ok
+    flags, recs = mesh.preprocess(dims, spacing, internal * O.FLAG_INTERNAL)
+    n = int(np.prod(dims))
+    flow = np.full(27, 0.001, np.float32)
+    flow[6] = 0.002
+    f = np.tile(flow[:, None], (1, n)).reshape(-1).copy()
+    return flags, recs, flow, f
+
+
+@pytest.mark.parametrize("mesh_name,tau", [("compat_cube.obj", 0.6),
+                                           ("compat_cube.obj", 1.0),
+                                           ("cube_angled.obj", 0.6)])
+def test_p1_reference_main_loop_1000_steps(data_dir, mesh_name, tau):
+    # LatticeBoltzmannCuda.cpp:29-169: f = 0.001 (f6 = 0.002), plane x = 0
+    # injection of the same vector each step, obstacle, force on.
+    say(f"case: P1 reference main loop 32^3 {mesh_name} tau={tau} x 1000")
+    dims, spacing, steps = (32, 32, 32), 0.05, 1000
+    flags, recs, flow, f = _p1_setup(data_dir, dims, spacing, mesh_name)
+    node, dirs, w = recs
+    assert len(node) > 0
+    with ref_ctx(dims, tau=tau) as c:
+        c.set_populations(f)
+        c.upload_flags(flags)
+        c.upload_reflections(node, dirs, w)
+        c.set_flow(capi.FLOW_PLANE, [0, 0], flow)
+        c.set_force([0, 0, 0])
+        c.step_reference(steps, with_force=True)
+        g = c.populations()
+        Fg = c.force()
+        rho_g, u_g = c.moments()
+    Fo = np.zeros(3, np.float32)
+    Fo64 = np.zeros(3, np.float64)
+    o = O.ref_run_main(dims, f, flags, recs, tau, O.FLOW_PLANE, [0, 0], flow,
+                       Fo, steps, Fo64)
+    rho_o, u_o = O.ref_moments(dims, o)
+    say(f"  f rel err {rel_err(g, o):.2e}  F gpu {Fg} exact {Fo64} "
+        f"float-per-term {Fo}")
+    assert rel_err(g, o) <= TOL
+    assert rel_err(rho_g, rho_o) <= TOL
+    assert vel_err(u_g, u_o) <= TOL
+    # The force register holds the sum of every reflection term.  The GPU
+    # adds one FP64-summed total per pass; it matches the exact sum.
+    assert rel_err(Fg, Fo64) <= TOL
+    # The reference adds every term to a float with atomics (order
+    # unspecified).  With ~1e5 terms of ~1e-5 per pass into a total of
+    # ~1e3 that accumulation drops low-order bits; the oracle's float
+    # accumulation in node order shows the size of that drift.
+    assert rel_err(Fo, Fo64) <= 0.05
+    # internal nodes hold nothing after the reflection stage + collision
+    internal = (flags & O.FLAG_INTERNAL) != 0
+    assert np.abs(g.reshape(27, -1)[:, internal]).max() == 0.0
+
+
+def test_reference_staged_equals_fused_full_step(data_dir):
+    say("case: staged S,R,C,F == lbm_step_reference")
+    dims, spacing = (20, 20, 20), 0.05
+    flags, recs, flow, f = _p1_setup(data_dir, dims, spacing, "cube_angled.obj")
+    outs = []
+    for staged in (True, False):
+        with ref_ctx(dims, tau=0.7) as c:
+            c.set_populations(f)
+            c.upload_flags(flags)
+            c.upload_reflections(*recs)
+            c.set_flow(capi.FLOW_PLANE, [0, 0], flow)
+            c.set_force([0, 0, 0])
+            for _ in range(20):
+                if staged:
+                    c.stream()
+                    c.reflect(True)
+                    c.collide(0.7)
+                    c.flow()
+                else:
+                    c.step_reference(1, True)
+            outs.append((c.populations(), c.force()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
+# ------------------------------------------------ standard mode vs oracle
+def _std_ctx(q, dims, storage=capi.STORE_F32, **kw):
+    return capi.Context(capi.make_config(*dims, lattice=q, storage=storage,
+                                         **kw))
+
+
+@pytest.mark.parametrize("q,dims", [(19, (32, 32, 32)), (9, (128, 128, 1))])
+def test_c1_cavity_1000_steps(q, dims):
+    # lid-driven cavity: lid = y max face moving at (0.1, 0, 0), half-way
+    # bounce-back on every other face, tau = 0.6, rho = 1, u = 0
+    say(f"case: C1 cavity D{3 if q != 9 else 2}Q{q} {dims} x 1000")
+    tau, steps, lid = 0.6, 1000, 3
+    kw = dict(tau=tau, lid_face=lid, lid_u=(0.1, 0, 0), periodic=(0, 0, 0))
+    with _std_ctx(q, dims, **kw) as c:
+        c.init_equilibrium()
+        mask_g = c.mask()
+        c.step(steps)
+        fg = c.populations()
+        rho_g, u_g = c.moments()
+    mask_o = O.std_build_mask(q, dims, None, (0, 0, 0), lid)
+    assert np.array_equal(mask_g, mask_o)
+    f0 = O.std_init_eq(q, dims)
+    p = O.std_params(tau=tau, lid_face=lid, lid_u=(0.1, 0, 0))
+    fo = O.std_run_f32(q, dims, f0, mask_o, p, steps)
+    rho_o, u_o = O.std_moments(q, dims, fo, mask_o)
+    say(f"  f rel err {rel_err(fg, fo):.2e} u err {vel_err(u_g, u_o):.2e} "
+        f"max|u| {np.abs(u_o).max():.3e}")
+    assert rel_err(fg, fo) <= TOL
+    assert rel_err(rho_g, rho_o) <= TOL
+    assert vel_err(u_g, u_o) <= TOL
+
+
+def test_c2_channel_small_vs_oracle():
+    # the headline configuration at reduced size: periodic x, walls y/z,
+    # random spheres to 20% solid (seed 2), Guo force 1e-6 along x, tau 0.8
+    say("case: C2 channel 64x32x32 x 300")
+    dims, tau, steps = (64, 32, 32), 0.8, 300
+    solid, _ = capi.random_spheres(dims, 2, 0.2, 4.0, 8.0, True)
+    kw = dict(tau=tau, periodic=(1, 0, 0), force=(1e-6, 0, 0))
+    with _std_ctx(19, dims, **kw) as c:
+        c.upload_solid(solid)
+        c.init_equilibrium()
+        mask_g = c.mask()
+        c.step(steps)
+        fg = c.populations()
+        rho_g, u_g = c.moments()
+    mask_o = O.std_build_mask(19, dims, solid, (1, 0, 0))
+    assert np.array_equal(mask_g, mask_o)
+    p = O.std_params(tau=tau, force=(1e-6, 0, 0), periodic=(1, 0, 0))
+    fo = O.std_run_f32(19, dims, O.std_init_eq(19, dims), mask_o, p, steps)
+    rho_o, u_o = O.std_moments(19, dims, fo, mask_o, (1e-6, 0, 0))
+    fl = solid == 0
+    say(f"  f rel err {rel_err(fg, fo):.2e} u err {vel_err(u_g, u_o):.2e}")
+    assert rel_err(fg, fo) <= TOL  # solid nodes hold w_i on both sides
+    assert rel_err(rho_g, rho_o) <= TOL
+    assert vel_err(u_g, u_o) <= TOL
+    assert u_o[0][fl].mean() > 0  # the force drives flow along +x
+    # At |u| ~ 1e-5 FP32 rounding dominates the velocity.  Both FP32 paths
+    # are equally far from the FP64 solution: the GPU is as accurate as the
+    # oracle's FP32 run.
+    f64 = O.std_run_f64(19, dims, O.std_init_eq(19, dims), mask_o, p, steps)
+    _, u64 = O.std_moments(19, dims, f64, mask_o, (1e-6, 0, 0))
+    e_gpu = np.abs(u_g - u64).max()
+    e_orc = np.abs(u_o - u64).max()
+    say(f"  |u - u_fp64|: gpu {e_gpu:.2e} oracle fp32 {e_orc:.2e}")
+    assert e_gpu <= 2.0 * e_orc + 1e-9
+
+
+@pytest.mark.parametrize("storage,tol", [(capi.STORE_F64, 1e-12),
+                                         (capi.STORE_F32, 2e-5)])
+def test_c2_full_size_mask_and_mass(storage, tol):
+    # 256x128x128: link masks bit-exact with the oracle; the fluid mass
+    # (FP64 sum) is conserved by the step.  Exactly in FP64 storage; FP32
+    # storage rounds every population every step and drifts by ~2e-8 per
+    # step (the oracle's FP32 run drifts the same way).
+    say(f"case: C2 channel full size 256x128x128 storage {storage} x 200")
+    dims = (256, 128, 128)
+    solid, sph = capi.random_spheres(dims, 2, 0.2, 4.0, 8.0, True)
+    assert solid.mean() >= 0.2
+    kw = dict(tau=0.8, periodic=(1, 0, 0), force=(1e-6, 0, 0))
+    with _std_ctx(19, dims, storage=storage, **kw) as c:
+        c.upload_solid(solid)
+        c.init_equilibrium()
+        m_g = c.mask()
+        mo = O.std_build_mask(19, dims, solid, (1, 0, 0))
+        assert np.array_equal(m_g, mo)
+        c0 = c.conservation()
+        c.step(200)
+        c1 = c.conservation()
+    fluid = float((solid == 0).sum())
+    assert abs(c0[0] - fluid) < 1e-6 * fluid
+    say(f"  mass drift {(c1[0] - c0[0]) / c0[0]:.2e}")
+    assert abs(c1[0] - c0[0]) < tol * c0[0]
+    assert c1[1] > 0  # net momentum along the force
+
+
+def test_std_f64_matches_oracle_f64():
+    say("case: FP64 GPU vs FP64 oracle 24x20x16 x 200")
+    dims, q = (24, 20, 16), 19
+    rng = np.random.default_rng(4)
+    solid = (rng.random(int(np.prod(dims))) < 0.1).astype(np.uint8)
+    kw = dict(tau=0.7, periodic=(1, 0, 0), force=(1e-5, 0, 0), lid_face=3,
+              lid_u=(0.05, 0, 0))
+    with _std_ctx(q, dims, storage=capi.STORE_F64, **kw) as c:
+        c.upload_solid(solid)
+        c.init_equilibrium()
+        c.step(200)
+        fg = c.populations(np.float64)
+    mask = O.std_build_mask(q, dims, solid, (1, 0, 0), 3)
+    p = O.std_params(tau=0.7, force=(1e-5, 0, 0), lid_face=3,
+                     lid_u=(0.05, 0, 0), periodic=(1, 0, 0))
+    fo = O.std_run_f64(q, dims, O.std_init_eq(q, dims), mask, p, 200)
+    assert np.abs(fg - fo).max() < 1e-12
+
+
+@pytest.mark.parametrize("storage,oracle_storage,tol",
+                         [(capi.STORE_F16, O.STORE_F16_SHIFTED, 2e-3),
+                          (capi.STORE_BF16, O.STORE_BF16_SHIFTED, 2e-2)])
+def test_shifted_storage_tracks_oracle_emulation(storage, oracle_storage, tol):
+    say(f"case: shifted storage {storage} vs oracle emulation")
+    dims, q = (32, 24, 16), 19
+    kw = dict(tau=0.8, periodic=(1, 0, 0), lid_face=3, lid_u=(0.05, 0, 0))
+    with _std_ctx(q, dims, storage=storage, **kw) as c:
+        c.init_equilibrium()
+        c.step(100)
+        fg = c.populations(np.float64)
+        _, ug = c.moments()
+    mask = O.std_build_mask(q, dims, None, (1, 0, 0), 3)
+    p = O.std_params(tau=0.8, lid_face=3, lid_u=(0.05, 0, 0),
+                     periodic=(1, 0, 0))
+    f0 = O.std_init_eq(q, dims)
+    fo = O.std_run_f32(q, dims, f0, mask, p, 100, oracle_storage)
+    _, uo = O.std_moments(q, dims, fo, mask)
+    # the same rounding rule on both sides: differences only where an
+    # FMA-contracted float result lands on the other side of a rounding
+    # boundary
+    du = np.abs(ug[0] - uo[0]).max() / np.abs(uo[0]).max()
+    say(f"  u rel diff {du:.2e}")
+    assert du < tol
+
+
+def test_conservation_and_moments_api():
+    say("case: moments / conservation API")
+    dims, q = (16, 12, 10), 19
+    rng = np.random.default_rng(9)
+    n = int(np.prod(dims))
+    rho = 1 + 0.01 * rng.standard_normal(n)
+    u = 0.01 * rng.standard_normal((3, n))
+    with _std_ctx(q, dims, tau=0.9, periodic=(1, 1, 1)) as c:
+        c.init_equilibrium(rho, u)
+        r, uu = c.moments()
+        cons = c.conservation()
+    assert np.allclose(r, rho, rtol=1e-6)
+    assert np.allclose(uu, u, atol=1e-6)
+    assert abs(cons[0] - rho.sum()) < 1e-9 * n
+    assert np.allclose(cons[1:], (rho * u).sum(1), atol=1e-8 * n)

@@ -4,6 +4,9 @@
 
 produces, in-tree under latticeboltzmanncuda_amd/lib/:
   liblbm_hip.so   the C-ABI of include/lbm.h (HIP kernels for gfx950)
+  liblbm.so       the host-only C++ facade of include/lbm/*.h (Lattice,
+                  Model, Geometry, FlowData ...) over that C-ABI (g++)
+and under latticeboltzmanncuda_amd/bin/ the C++ test suites of tests/cpp/.
 
 Every .hip file is compiled to an object with hipcc (parallel), then
 linked into one shared library.  Compilation always runs (no staleness
@@ -41,6 +44,16 @@ HIP_FLAGS = [
 
 HIP_SOURCES = ["lbm_capi.hip", "k_standard.hip", "k_reference.hip",
                "k_common.hip"]
+HOST_SOURCES = ["Constants.cpp", "Geometry.cpp", "Utils.cpp",
+                "FlowCriterion.cpp", "Model.cpp", "Lattice.cpp"]
+CXX = os.environ.get("CXX", "g++")
+# portable x86-64 code: the library is built here and runs on the GPU host
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-march=x86-64-v2",
+             "-Wall", f"-I{INCLUDE}", f"-I{os.path.join(INCLUDE, 'lbm')}"]
+BINDIR = os.path.join(PKG, "bin")
+TESTS_CPP = os.path.join(ROOT, "tests", "cpp")
+CPP_SUITES = ["GeometryTest", "ModelTest", "CalculationsTest",
+              "CoordinatesTest", "GeometryDump"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -76,10 +89,48 @@ def build_hip(jobs: int | None = None) -> str:
     return out
 
 
+def build_facade() -> str:
+    """liblbm.so: the C++ facade, linked against liblbm_hip.so."""
+    os.makedirs(OBJDIR, exist_ok=True)
+    objs = []
+    for src in HOST_SOURCES:
+        obj = os.path.join(OBJDIR, "host_" + os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+    with cf.ThreadPoolExecutor(len(HOST_SOURCES)) as ex:
+        list(ex.map(lambda so: _run([CXX, *CXX_FLAGS, "-c",
+                                     os.path.join(HOST, so[0]), "-o", so[1]]),
+                    zip(HOST_SOURCES, objs)))
+    out = os.path.join(LIBDIR, "liblbm.so")
+    tmp = out + ".tmp"
+    _run([CXX, "-shared", "-fopenmp", *objs, f"-L{LIBDIR}", "-llbm_hip",
+          "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_cpp_suites() -> list[str]:
+    """The C++ suites of tests/cpp (the reference's four gtest suites
+    re-expressed on the facade)."""
+    os.makedirs(BINDIR, exist_ok=True)
+
+    def one(name: str) -> str:
+        out = os.path.join(BINDIR, name)
+        _run([CXX, *CXX_FLAGS, f"-I{TESTS_CPP}",
+              os.path.join(TESTS_CPP, name + ".cpp"), f"-L{LIBDIR}", "-llbm",
+              "-llbm_hip", "-Wl,-rpath,$ORIGIN/../lib", "-o", out + ".tmp"])
+        os.replace(out + ".tmp", out)
+        return out
+
+    with cf.ThreadPoolExecutor(len(CPP_SUITES)) as ex:
+        return list(ex.map(one, CPP_SUITES))
+
+
 def build_all() -> list[str]:
     if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
-    return [build_hip()]
+    out = [build_hip(), build_facade()]
+    out += build_cpp_suites()
+    return out
 
 
 if __name__ == "__main__":

@@ -7,14 +7,16 @@
 // lattice shape); (x, y, z) come from two magic-number divisions.
 //
 // The step, for every node:
-//   1. pulls f_i = A_i(x - e_i) for all i with periodic wrap -- issued
-//      unconditionally, so the loads never wait for the geometry;
-//   2. boundary nodes (2-bit class map, L2-resident) read their link mask
-//      and replace each bounced link by the half-way bounce-back value
-//      A_inv(i)(x) (+ the moving-wall term 6 w_i rho_w e_i.u_w when the
-//      source lies beyond the moving face);
+//   1. reads its class from a 2-bit class map (1 MB at 256x128x128, L2
+//      resident): interior fluid, boundary fluid, or solid;
+//   2. fluid nodes pull f_i = A_i(x - e_i) with periodic wrap; boundary
+//      nodes first read their link mask and, for each bounced link, load
+//      the half-way bounce-back value A_inv(i)(x) instead (+ the moving-wall
+//      term 6 w_i rho_w e_i.u_w when the source lies beyond the moving
+//      face) -- one load per link, the address is selected, so no solid
+//      data is fetched; solid nodes load nothing;
 //   3. collide (BGK + Guo forcing) in FP32 (FP64 for F64 storage);
-//   4. stores every direction with all lanes active; solid nodes store the
+//   4. store every direction with all lanes active; solid nodes store the
 //      rest state w_i.  Skipping the stores of solid lanes leaves partly
 //      written cache lines that cost ~1.7x in time on MI355X.
 // Consecutive steps walk the tiles in opposite orders so that the start of
@@ -64,31 +66,30 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t sz[3] = {z + 1 == g.nz ? 0u : z + 1, z,
                           (z == 0 ? g.nz : z) - 1};
 
+  const bool solid = c == kClassSolid;
+  const uint32_t m = c == kClassBoundary ? mask[n] : 0u;
   C f[Q];
+  if (!solid) {
 #pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const uint32_t src = sx[L::E[i][0] + 1] + sy[L::E[i][1] + 1] + sz[L::E[i][2] + 1];
-    f[i] = St::load(A + pidx<Q>(src, i));
-  }
-  if (c == kClassBoundary) {
-    const uint32_t m = mask[n];
-#pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      if ((m >> i) & 1u) {
-        C v = St::load(A + pidx<Q>(n, L::INV[i]));
-        if (LID) {
-          const int ex = L::E[i][0], ey = L::E[i][1], ez = L::E[i][2];
-          const int el = p.lid_axis == 0 ? ex : (p.lid_axis == 1 ? ey : ez);
-          if ((m & LBM_MASK_LID) && el == -p.lid_sign) {
-            const C eu = (C)ex * (C)p.lid_u[0] + (C)ey * (C)p.lid_u[1] +
-                         (C)ez * (C)p.lid_u[2];
-            // shifted storage: the wall term is added to f, not to f - w
-            v += (C)6 * (C)L::W[i] * (C)p.wall_rho * eu;
-          }
+    for (int i = 0; i < Q; ++i) {
+      const uint32_t src = sx[L::E[i][0] + 1] + sy[L::E[i][1] + 1] + sz[L::E[i][2] + 1];
+      const bool bb = (m >> i) & 1u;
+      C v = St::load(A + (bb ? pidx<Q>(n, L::INV[i]) : pidx<Q>(src, i)));
+      if (LID) {
+        const int ex = L::E[i][0], ey = L::E[i][1], ez = L::E[i][2];
+        const int el = p.lid_axis == 0 ? ex : (p.lid_axis == 1 ? ey : ez);
+        if (bb && (m & LBM_MASK_LID) && el == -p.lid_sign) {
+          const C eu = (C)ex * (C)p.lid_u[0] + (C)ey * (C)p.lid_u[1] +
+                       (C)ez * (C)p.lid_u[2];
+          // shifted storage: the wall term is added to f, not to f - w
+          v += (C)6 * (C)L::W[i] * (C)p.wall_rho * eu;
         }
-        f[i] = v;
       }
+      f[i] = v;
     }
+  } else {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = St::kShifted ? (C)0 : (C)L::W[i];
   }
   if (St::kShifted) {
 #pragma unroll
@@ -113,7 +114,6 @@ __global__ void __launch_bounds__(kBlock)
   const C usq = ux * ux + uy * uy + uz * uz;
   const C uF = ux * Fx + uy * Fy + uz * Fz;
   const C base = (C)1 - (C)1.5 * usq;
-  const bool solid = c == kClassSolid;
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const C ex = (C)L::E[i][0], ey = (C)L::E[i][1], ez = (C)L::E[i][2];

@@ -1,0 +1,330 @@
+// Host-side Lattice facade over the C-ABI (reference:
+// simulation/source/Lattice.cpp:22-477).  The lattice is a reference-mode
+// D3Q27 context of liblbm_hip.so; geometry preprocessing (insertModel,
+// preProcessModel) runs on the host as in the reference and uploads flag
+// bytes and reflection records.
+#include "Lattice.h"
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "Constants.h"
+#include "Model.h"
+#include "lbm.h"
+
+namespace lbm {
+void check(int rc, const char* where) {
+  if (rc != LBM_OK) throw Error(rc, std::string(where) + ": " + lbm_last_error());
+}
+}  // namespace lbm
+
+namespace {
+
+constexpr unsigned kFieldMask = 0x1Fu;
+
+inline int64_t node_index(int x, int y, int z, int ny, int nz) {
+  return (int64_t)z + (int64_t)nz * ((int64_t)y + (int64_t)ny * x);
+}
+
+inline bool in_box(int x, int y, int z, int nx, int ny, int nz) {
+  return x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz;
+}
+
+// 5-bit slot fields, slot s in word s/6 at bit 5*(s%6)
+// (Lattice.cpp:450-477, SimCalcFuncs.cu:18-30).
+inline void set_slot(int* words, int s, unsigned value) {
+  const unsigned shift = 5u * (unsigned)(s % 6);
+  unsigned w = (unsigned)words[s / 6];
+  w = (w & ~(kFieldMask << shift)) | ((value & kFieldMask) << shift);
+  words[s / 6] = (int)w;
+}
+
+inline int get_slot(const int* words, int s) {
+  return (int)(((unsigned)words[s / 6] >> (5u * (unsigned)(s % 6))) & kFieldMask);
+}
+
+}  // namespace
+
+Lattice::Lattice(int x, int y, int z, dim3 blocks, dim3 threads,
+                 FluidData fluid, float spacing)
+    : m_fluid(fluid),
+      m_xResolution(x),
+      m_yResolution(y),
+      m_zResolution(z),
+      m_latticeSpacing(spacing),
+      m_threads(threads),
+      m_blocks(blocks) {
+  if ((int64_t)blocks.x * threads.x != x || (int64_t)blocks.y * threads.y != y ||
+      (int64_t)blocks.z * threads.z != z)
+    throw std::invalid_argument(
+        "Lattice: blocks x threads must equal the lattice extents");
+  lbm_config cfg{};
+  cfg.nx = x;
+  cfg.ny = y;
+  cfg.nz = z;
+  cfg.lattice = LBM_D3Q27;
+  cfg.storage = LBM_STORE_F32;
+  cfg.mode = LBM_MODE_REFERENCE;
+  cfg.tau = fluid.m_characteristicTimescale;
+  cfg.lid_face = -1;
+  cfg.wall_rho = 1.0f;
+  lbm::check(lbm_create(&cfg, nullptr, &m_ctx), "lbm_create");
+}
+
+Lattice::~Lattice() { lbm_destroy(m_ctx); }
+
+void Lattice::load_data(LatticePoint* lattice_array) {
+  const int64_t n = (int64_t)m_xResolution * m_yResolution * m_zResolution;
+  lbm::check(lbm_upload_points(m_ctx, reinterpret_cast<const lbm_lattice_point*>(lattice_array), n),
+             "lbm_upload_points");
+}
+
+LatticePoint* Lattice::retrieve_data() {
+  const int64_t n = (int64_t)m_xResolution * m_yResolution * m_zResolution;
+  LatticePoint* out = new LatticePoint[n];
+  const int rc = lbm_download_points(m_ctx, reinterpret_cast<lbm_lattice_point*>(out), n);
+  if (rc != LBM_OK) {
+    delete[] out;
+    lbm::check(rc, "lbm_download_points");
+  }
+  for (std::size_t r = 0; r < m_reflectionNodes.size(); ++r)
+    out[m_reflectionNodes[r]].reflections = &m_reflections[r];
+  return out;
+}
+
+ReflectionData* Lattice::retrieve_reflection_data() {
+  if (!m_reflectionData) return nullptr;
+  float f[3];
+  lbm::check(lbm_get_force(m_ctx, f), "lbm_get_force");
+  m_reflectionData->x = f[0];
+  m_reflectionData->y = f[1];
+  m_reflectionData->z = f[2];
+  return m_reflectionData;
+}
+
+void Lattice::simulateStreaming() { lbm::check(lbm_stream(m_ctx), "lbm_stream"); }
+
+void Lattice::simulateCollision() {
+  lbm::check(lbm_collide(m_ctx, m_fluid.m_characteristicTimescale), "lbm_collide");
+}
+
+void Lattice::simulateReflections() {
+  lbm::check(lbm_reflect(m_ctx, m_reflectionData != nullptr ? 1 : 0), "lbm_reflect");
+}
+
+void Lattice::simulateFlow() {
+  if (m_flowData) lbm::check(lbm_flow(m_ctx), "lbm_flow");
+}
+
+void Lattice::simulateLattice() { lbm::check(lbm_step(m_ctx, 1), "lbm_step"); }
+
+std::vector<uint8_t> Lattice::downloadFlags() {
+  std::vector<uint8_t> flags((std::size_t)m_xResolution * m_yResolution * m_zResolution);
+  lbm::check(lbm_get_flags(m_ctx, flags.data()), "lbm_get_flags");
+  return flags;
+}
+
+namespace lbm {
+
+bool mark_internal(Model& model, int nx, int ny, int nz, float spacing,
+                   std::vector<uint8_t>& flags) {
+  if (!model.isLoaded()) return false;
+  const MeshBounds b = model.bounding_box();
+  if (b.xmin() < 0.0f || b.ymin() < 0.0f || b.zmin() < 0.0f ||
+      b.xmax() > nx * spacing || b.ymax() > ny * spacing ||
+      b.zmax() > nz * spacing)
+    return false;
+  flags.resize((std::size_t)nx * ny * nz, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z) {
+        const int64_t n = node_index(x, y, z, ny, nz);
+        const bool inside = model.isPointInsideModel(
+            Point_3((float)x * spacing, (float)y * spacing, (float)z * spacing));
+        flags[n] = (uint8_t)((flags[n] & ~LBM_FLAG_INTERNAL) |
+                             (inside ? LBM_FLAG_INTERNAL : 0u));
+      }
+  return true;
+}
+
+// score_j = 2 - |v - e_j/|e_j||, all 27 directions (rest included),
+// highest score first, equal scores by higher index first (the order of
+// std::priority_queue<std::pair<float, int>>).
+std::array<std::pair<float, int>, 27> distribute_vector(Point_3 vector) {
+  std::array<std::pair<float, int>, 27> out;
+  for (int j = 0; j < 27; ++j) {
+    const int* e = CONSTANTS::DIRECTIONS[j];
+    const int len2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+    const double inv = len2 == 0 ? 0.0 : (len2 == 1 ? 1.0 : (len2 == 2 ? 1.0 / 1.4142135623730951 : 1.0 / 1.7320508075688772));
+    float unit[3];
+    for (int a = 0; a < 3; ++a) unit[a] = e[a] == 0 ? 0.0f : (float)(e[a] * inv);
+    const float dx = vector.x - unit[0], dy = vector.y - unit[1], dz = vector.z - unit[2];
+    out[j] = {(float)(2.0 - (double)std::sqrt((dx * dx) + (dy * dy) + (dz * dz))), j};
+  }
+  std::sort(out.begin(), out.end(),
+            [](const std::pair<float, int>& a, const std::pair<float, int>& b) {
+              return a > b;
+            });
+  return out;
+}
+
+// For every internal node x and direction k with x + e_k inside the box and
+// not internal: reflect along the ray from (x + e_k) h to x h, rank the
+// directions, keep the first three whose target x + e_d is inside the box
+// and not internal (the reference reads outside its host array for targets
+// beyond the box; they are not eligible here), store them in slots
+// INVERSE[k]*3 + rank and normalise the three weights.
+void build_reflections(Model& model, int nx, int ny, int nz, float h,
+                       std::vector<uint8_t>& flags,
+                       std::deque<ReflectionValues>& records,
+                       std::vector<int32_t>& nodes) {
+  auto internal = [&](int x, int y, int z) {
+    return (flags[node_index(x, y, z, ny, nz)] & LBM_FLAG_INTERNAL) != 0;
+  };
+  records.clear();
+  nodes.clear();
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z) {
+        if (!internal(x, y, z)) continue;
+        ReflectionValues* rec = nullptr;
+        for (int k = 0; k < 27; ++k) {
+          const int* ek = CONSTANTS::DIRECTIONS[k];
+          const int sx = x + ek[0], sy = y + ek[1], sz = z + ek[2];
+          if (!in_box(sx, sy, sz, nx, ny, nz) || internal(sx, sy, sz)) continue;
+          const Point_3 v = model.reflectionVector(
+              Point_3(h * (float)sx, h * (float)sy, h * (float)sz),
+              Point_3(h * (float)x, h * (float)y, h * (float)z));
+          const std::array<std::pair<float, int>, 27> ranked = distribute_vector(v);
+          const int base = CONSTANTS::INVERSE_DIRECTIONS[k] * 3;
+          int counter = 0;
+          float total = 0.0f;
+          for (int i = 0; i < 27 && counter < 3; ++i) {
+            const int d = ranked[i].second;
+            const int* ed = CONSTANTS::DIRECTIONS[d];
+            const int tx = x + ed[0], ty = y + ed[1], tz = z + ed[2];
+            if (!in_box(tx, ty, tz, nx, ny, nz) || internal(tx, ty, tz)) continue;
+            if (!rec) {
+              records.emplace_back();
+              rec = &records.back();
+              std::memset(rec, 0, sizeof(ReflectionValues));
+              nodes.push_back((int32_t)node_index(x, y, z, ny, nz));
+            }
+            set_slot(rec->reflection_directions, base + counter, (unsigned)d);
+            rec->reflection_weight[base + counter] = ranked[i].first;
+            total += ranked[i].first;
+            ++counter;
+          }
+          if (total != 0.0f && rec)
+            for (int i = 0; i < 3; ++i)
+              if (get_slot(rec->reflection_directions, base + i) != 0)
+                rec->reflection_weight[base + i] /= total;
+        }
+        if (rec) flags[node_index(x, y, z, ny, nz)] |= LBM_FLAG_REFLECTED;
+      }
+}
+
+}  // namespace lbm
+
+// Lattice.cpp:228-260
+void Lattice::insertModel(std::string filename) {
+  m_simModel.reset(new Model());
+  m_simModel->importModel(filename);
+  std::vector<uint8_t> flags = downloadFlags();
+  if (!lbm::mark_internal(*m_simModel, m_xResolution, m_yResolution,
+                          m_zResolution, m_latticeSpacing, flags))
+    return;
+  lbm::check(lbm_upload_flags(m_ctx, flags.data()), "lbm_upload_flags");
+}
+
+std::array<std::pair<float, int>, 27> Lattice::distributeVector(Point_3 vector) {
+  return lbm::distribute_vector(vector);
+}
+
+// Lattice.cpp:323-426.  Records are rebuilt from scratch on every call.
+void Lattice::preProcessModel() {
+  if (!m_simModel) return;
+  std::vector<uint8_t> flags = downloadFlags();
+  lbm::build_reflections(*m_simModel, m_xResolution, m_yResolution,
+                         m_zResolution, m_latticeSpacing, flags,
+                         m_reflections, m_reflectionNodes);
+  const std::size_t nrec = m_reflections.size();
+  std::vector<uint32_t> dirs(14 * nrec);
+  std::vector<float> weights(81 * nrec);
+  for (std::size_t r = 0; r < nrec; ++r) {
+    std::memcpy(&dirs[14 * r], m_reflections[r].reflection_directions, 14 * sizeof(uint32_t));
+    std::memcpy(&weights[81 * r], m_reflections[r].reflection_weight, 81 * sizeof(float));
+  }
+  lbm::check(lbm_upload_flags(m_ctx, flags.data()), "lbm_upload_flags");
+  lbm::check(lbm_upload_reflections(m_ctx, (int32_t)nrec, m_reflectionNodes.data(),
+                                    dirs.data(), weights.data()),
+             "lbm_upload_reflections");
+}
+
+void Lattice::setFlowData(FlowData* flowData) {
+  m_flowData = flowData;
+  if (!flowData) {
+    lbm::check(lbm_set_flow(m_ctx, LBM_FLOW_NONE, nullptr, nullptr, nullptr), "lbm_set_flow");
+    return;
+  }
+  lbm::check(lbm_set_flow(m_ctx, (int)flowData->pointCriterion.kind,
+                          flowData->pointCriterion.params,
+                          flowData->inducedFlow.particle_distribution, nullptr),
+             "lbm_set_flow");
+}
+
+void Lattice::setReflectionData(ReflectionData* reflection) {
+  m_reflectionData = reflection;
+  if (reflection) {
+    const float f[3] = {reflection->x, reflection->y, reflection->z};
+    lbm::check(lbm_set_force(m_ctx, f), "lbm_set_force");
+  }
+}
+
+void Lattice::setReflectionDirection(LatticePoint* point, const int index,
+                                     const unsigned int value) {
+  set_slot(point->reflections->reflection_directions, index, value);
+}
+
+int Lattice::getReflectionDirection(LatticePoint* point, const int index) {
+  return get_slot(point->reflections->reflection_directions, index);
+}
+
+// ---------------------------------------------------------------- wrappers
+namespace RunHipFunctions {
+void run_calculate_streaming(dim3, dim3, LatticeData lattice, LatticeData) {
+  lbm::check(lbm_stream(lattice.latticePtr), "lbm_stream");
+}
+void run_calculate_collision(dim3, dim3, LatticeData lattice, float timescale) {
+  lbm::check(lbm_collide(lattice.latticePtr, timescale), "lbm_collide");
+}
+void run_calculate_reflections(dim3, dim3, LatticeData lattice, LatticeData) {
+  lbm::check(lbm_reflect(lattice.latticePtr, 0), "lbm_reflect");
+}
+void run_calculate_reflections_data(dim3, dim3, LatticeData lattice, LatticeData,
+                                    ReflectionData*) {
+  lbm::check(lbm_reflect(lattice.latticePtr, 1), "lbm_reflect");
+}
+void run_prime_points(dim3, dim3, LatticeData lattice) {
+  lbm::check(lbm_prime_points(lattice.latticePtr), "lbm_prime_points");
+}
+void run_generate_flow(dim3, dim3, LatticeData lattice, FlowData*) {
+  lbm::check(lbm_flow(lattice.latticePtr), "lbm_flow");
+}
+}  // namespace RunHipFunctions
+
+namespace RunHipTestFunctions {
+void run_test_coordinates(dim3, dim3, LatticeHandle lattice, int, bool* result) {
+  int ok = 0;
+  lbm::check(lbm_test_coordinates(lattice, &ok), "lbm_test_coordinates");
+  if (!ok) *result = false;
+}
+void run_test_streaming(dim3, dim3, LatticeData lattice, LatticeData) {
+  lbm::check(lbm_stream(lattice.latticePtr), "lbm_stream");
+}
+void run_prime_points(dim3, dim3, LatticeData lattice) {
+  lbm::check(lbm_prime_points(lattice.latticePtr), "lbm_prime_points");
+}
+}  // namespace RunHipTestFunctions

@@ -550,6 +550,8 @@ void orc_mesh_reflection_vector(const orc_mesh* m, const float p1[3],
   const double o[3] = {p1[0], p1[1], p1[2]};
   const double d[3] = {(double)p2[0] - p1[0], (double)p2[1] - p1[1],
                        (double)p2[2] - p1[2]};
+  /* nearest hit; hits within 1e-9 of the nearest (a ray through an edge
+   * or vertex shared by several faces) go to the lowest face index */
   int best = -1;
   double bt = 0.0;
   double a[3], b[3], c[3];
@@ -557,11 +559,18 @@ void orc_mesh_reflection_vector(const orc_mesh* m, const float p1[3],
     tri_d(m, t, a, b, c);
     double tt;
     int edge;
-    if (ray_tri(o, d, a, b, c, 0.0, &tt, &edge) && (best < 0 || tt < bt)) {
-      best = t;
-      bt = tt;
-    }
+    if (ray_tri(o, d, a, b, c, 0.0, &tt, &edge) && (best < 0 || tt < bt)) bt = tt, best = t;
   }
+  if (best >= 0)
+    for (int t = 0; t < best; ++t) {
+      tri_d(m, t, a, b, c);
+      double tt;
+      int edge;
+      if (ray_tri(o, d, a, b, c, 0.0, &tt, &edge) && tt <= bt + 1e-9) {
+        best = t;
+        break;
+      }
+    }
   if (best < 0) {
     out[0] = out[1] = out[2] = 0.0f;
     return;

@@ -74,6 +74,21 @@ def cpu_baseline(solid: np.ndarray, seconds: float) -> dict:
     }
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles",
+                            "k_std_step_d3q19_f32_channel_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the step kernel from the committed
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs (calibrated on the copy
+    kernel), or None when no such measurement is committed."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            return json.load(fh)["traffic_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -86,6 +101,8 @@ def main():
                                          True)
     n = int(np.prod(DIMS))
     fluid = int((solid == 0).sum())
+    # practical HBM ceiling: a plain copy of one population array
+    copy_gbs = capi.measure_copy_bandwidth(n * 19 * 4, 20)
     cfg = capi.make_config(*DIMS, lattice=capi.D3Q19,
                            storage=capi.STORE_F32, tau=TAU,
                            periodic=(1, 0, 0), force=FORCE)
@@ -139,9 +156,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(),
             "kernel": "k_std_step<19, f32>",
             "bytes_per_launch": bytes_per_launch,
+            "copy_gbs": round(copy_gbs, 1),
+            "frac_of_copy": round(achieved / copy_gbs, 4),
         },
     }
     ctx.close()

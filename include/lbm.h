@@ -13,8 +13,9 @@
  *   - 144-byte LatticePoint host records (simulation/include/LatticePoint.h:3-17)
  *     as the exchange format of Lattice::load_data / retrieve_data.
  *
- * Device layout (not visible through the ABI): structure of arrays, one
- * plane of N populations per lattice direction, double-buffered.
+ * Device layout (not visible through the ABI): blocked structure of
+ * arrays -- tiles of 64 consecutive nodes, inside a tile one contiguous run
+ * of 64 values per lattice direction -- double-buffered.
  *
  * The reference's launch boundary is namespace RunCudaFunctions
  * (simulation/include/SimCalcFuncs.cuh:13-26, SimCalcFuncs.cu:218-249),
@@ -227,6 +228,13 @@ int lbm_conservation(lbm_ctx* ctx, double out[4]);
  * (test/CudaTestHelper.cu:12-31): a device kernel checks that the launch
  * mapping visits every node once and that index <-> (x,y,z) round-trips. */
 int lbm_test_coordinates(lbm_ctx* ctx, int* ok);
+
+/* ------------------------------------------------------------ measurement */
+/* No counterpart in the reference: copies a `bytes`-sized device buffer
+ * into another with 16-byte accesses, iters times, and reports the
+ * read+write bandwidth in GB/s -- the practical HBM ceiling of the GPU,
+ * used by the benchmark beside the vendor peak. */
+int lbm_measure_copy_bandwidth(int64_t bytes, int iters, double* gbs);
 
 /* ------------------------------------------------------------ setup */
 /* Random spheres of the channel benchmark: draws (cx, cy, cz, r) from

@@ -119,6 +119,8 @@ _SIGNATURES = {
     "lbm_moments": (C.c_int, [CTX, P, P]),
     "lbm_conservation": (C.c_int, [CTX, P]),
     "lbm_test_coordinates": (C.c_int, [CTX, C.POINTER(C.c_int)]),
+    "lbm_measure_copy_bandwidth": (C.c_int, [i64, C.c_int,
+                                             C.POINTER(C.c_double)]),
     "lbm_random_spheres": (i64, [i32, i32, i32, C.c_uint64, C.c_double,
                                  C.c_double, C.c_double, i32, P, i64, P]),
 }
@@ -359,6 +361,14 @@ class Context:
 
     def device_bytes(self) -> int:
         return int(self.lib.lbm_device_bytes(self.h))
+
+
+def measure_copy_bandwidth(nbytes: int, iters: int = 20) -> float:
+    """GB/s (read + write) of a plain device-to-device copy of nbytes."""
+    g = C.c_double(0)
+    check(load().lbm_measure_copy_bandwidth(nbytes, iters, C.byref(g)),
+          "lbm_measure_copy_bandwidth")
+    return g.value
 
 
 def random_spheres(dims, seed, frac, rmin=4.0, rmax=8.0, periodic_x=True,

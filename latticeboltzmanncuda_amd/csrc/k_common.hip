@@ -100,9 +100,26 @@ __global__ void k_test_coords_check(Grid g, const uint32_t* __restrict__ counts,
   if (counts[n] != 1u) atomicAnd(ok, 0);
 }
 
+// Plain 16-byte-per-lane copy: the bandwidth reference of this GPU.
+__global__ void __launch_bounds__(kBlock)
+    k_copy4(const float4* __restrict__ a, float4* __restrict__ b, uint64_t n4) {
+  const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n4) b[i] = a[i];
+}
+
 inline unsigned blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 }  // namespace
+
+int launch_copy4(const void* src, void* dst, uint64_t bytes, hipStream_t s) {
+  const uint64_t n4 = bytes / 16;
+  const uint64_t nb = (n4 + kBlock - 1) / kBlock;
+  if (nb == 0 || nb > 0x7fffffffull) return fail(LBM_ERR_INVALID, "copy size out of range");
+  hipLaunchKernelGGL(k_copy4, dim3((unsigned)nb), dim3(kBlock), 0, s,
+                     (const float4*)src, (float4*)dst, n4);
+  LBM_HIP(hipGetLastError());
+  return LBM_OK;
+}
 
 
 int launch_points_in(int q, int storage, const Grid& g,

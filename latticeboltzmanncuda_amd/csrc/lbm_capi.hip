@@ -727,6 +727,43 @@ int lbm_test_coordinates(lbm_ctx* c, int* ok) {
   return d2h(c, ok, dok.p, sizeof(int));
 }
 
+int lbm_measure_copy_bandwidth(int64_t bytes, int iters, double* gbs) {
+  if (bytes < 16 || iters < 1 || !gbs)
+    return fail(LBM_ERR_INVALID, "bad copy-bandwidth arguments");
+  const uint64_t nbytes = (uint64_t)bytes / 16 * 16;
+  DevBuf a, b;
+  int rc;
+  if ((rc = a.alloc(nbytes)) || (rc = b.alloc(nbytes))) return rc;
+  hipStream_t s;
+  LBM_HIP(hipStreamCreate(&s));
+  hipEvent_t e0, e1;
+  hipError_t e;
+  if ((e = hipEventCreate(&e0)) != hipSuccess) {
+    (void)hipStreamDestroy(s);
+    return hip_fail(e, "hipEventCreate");
+  }
+  if ((e = hipEventCreate(&e1)) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    (void)hipStreamDestroy(s);
+    return hip_fail(e, "hipEventCreate");
+  }
+  float ms = 0.0f;
+  rc = LBM_OK;
+  if ((e = hipMemsetAsync(a.p, 0, nbytes, s)) != hipSuccess) rc = hip_fail(e, "hipMemsetAsync");
+  for (int w = 0; w < 3 && rc == LBM_OK; ++w) rc = launch_copy4(a.p, b.p, nbytes, s);
+  if (rc == LBM_OK && (e = hipEventRecord(e0, s)) != hipSuccess) rc = hip_fail(e, "hipEventRecord");
+  for (int k = 0; k < iters && rc == LBM_OK; ++k) rc = launch_copy4(a.p, b.p, nbytes, s);
+  if (rc == LBM_OK && (e = hipEventRecord(e1, s)) != hipSuccess) rc = hip_fail(e, "hipEventRecord");
+  if (rc == LBM_OK && (e = hipEventSynchronize(e1)) != hipSuccess) rc = hip_fail(e, "hipEventSynchronize");
+  if (rc == LBM_OK && (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) rc = hip_fail(e, "hipEventElapsedTime");
+  (void)hipStreamSynchronize(s);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  if (rc == LBM_OK) *gbs = 2.0 * (double)nbytes * iters / (ms * 1e-3) / 1e9;
+  return rc;
+}
+
 int64_t lbm_random_spheres(int32_t nx, int32_t ny, int32_t nz, uint64_t seed,
                            double frac, double rmin, double rmax,
                            int32_t periodic_x, uint8_t* solid,

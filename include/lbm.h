@@ -205,6 +205,10 @@ int lbm_get_force(lbm_ctx* ctx, float out[3]);
 /* n fused steps.  Standard mode: BGK pull collide+stream.  Reference mode:
  * Lattice::simulateLattice (Lattice.cpp:222-226), stream then collide. */
 int lbm_step(lbm_ctx* ctx, int n);
+/* n steps of each of count standard-mode lattices, one lbm_step launch
+ * per lattice per step (the unbatched way to step many lattices; no
+ * counterpart in the reference). */
+int lbm_step_each(lbm_ctx* const* ctxs, int32_t count, int n);
 /* Reference mode: n steps of the LatticeBoltzmannCuda.cpp:161-169 loop
  * (stream, reflect, collide, flow). */
 int lbm_step_reference(lbm_ctx* ctx, int n, int with_force);
@@ -228,6 +232,95 @@ int lbm_conservation(lbm_ctx* ctx, double out[4]);
  * (test/CudaTestHelper.cu:12-31): a device kernel checks that the launch
  * mapping visits every node once and that index <-> (x,y,z) round-trips. */
 int lbm_test_coordinates(lbm_ctx* ctx, int* ok);
+
+/* ------------------------------------------------------------ batches */
+/* No counterpart in the reference (it steps one lattice with one launch
+ * per stage).  A batch holds many standard-mode lattices of one storage
+ * type (D2Q9 / D3Q19 / D3Q27 may be mixed) that stay resident on the GPU;
+ * one persistent kernel per step updates all of them, and chunks of steps
+ * can be replayed from a captured hipGraph.  Admission is refused with
+ * LBM_ERR_BUDGET when the batch's device memory would exceed its budget. */
+typedef struct lbm_batch lbm_batch;
+int lbm_batch_create(int storage, int64_t budget_bytes, void* stream,
+                     lbm_batch** out);
+int lbm_batch_destroy(lbm_batch* b);
+/* device bytes lbm_batch_add would reserve for cfg */
+int64_t lbm_batch_estimate_bytes(const lbm_config* cfg, int track_residual);
+/* Adds a lattice (solid may be NULL), initialised to rho = 1, u = 0;
+ * track_residual keeps a velocity snapshot for lbm_batch_residuals. */
+int lbm_batch_add(lbm_batch* b, const lbm_config* cfg, const uint8_t* solid,
+                  int track_residual, int32_t* slot);
+int lbm_batch_remove(lbm_batch* b, int32_t slot);
+/* The member's context, for the per-lattice calls (init, populations,
+ * moments, conservation).  Do not lbm_step or lbm_destroy a member. */
+int lbm_batch_member(lbm_batch* b, int32_t slot, lbm_ctx** ctx);
+int lbm_batch_info(const lbm_batch* b, int32_t* lattices, int64_t* nodes,
+                   int64_t* device_bytes);
+/* Launch shape: block in {64,128,256,512} nodes per tile; grid = number of
+ * persistent workgroups, 0 = one workgroup per tile. */
+int lbm_batch_set_launch(lbm_batch* b, int block, int grid);
+int lbm_batch_get_launch(const lbm_batch* b, int* block, int* grid);
+/* Times every (block, grid) candidate over `steps` steps of the resident
+ * set and keeps the fastest; results are cached per (storage, lattice mix,
+ * log2 nodes) when use_cache.  The members advance by the steps run. */
+int lbm_batch_autotune(lbm_batch* b, int steps, int use_cache, int* block,
+                       int* grid);
+/* n steps of every member: plain launches, or full chunks of `chunk`
+ * (even) steps replayed from a captured hipGraph plus plain launches for
+ * the rest. */
+int lbm_batch_step(lbm_batch* b, int n);
+int lbm_batch_step_graph(lbm_batch* b, int n, int chunk);
+int lbm_batch_synchronize(lbm_batch* b);
+/* hipEvent-timed n steps (chunk 0: plain launches). */
+int lbm_batch_time_steps(lbm_batch* b, int n, int chunk, float* ms);
+/* out[slot] = |u - u_prev|_2 / |u|_2 over fluid nodes of every tracked
+ * member since its previous call (the snapshot is then replaced); -1 for
+ * empty or untracked slots.  nslots >= the number of slots. */
+int lbm_batch_residuals(lbm_batch* b, double* out, int32_t nslots);
+
+/* ------------------------------------------------------------ serving */
+/* No counterpart in the reference.  Continuous batching of solve requests
+ * on one GPU: requests arrive at arrival_s (seconds after the call
+ * starts), wait in FIFO order, are admitted into one batch while the HBM
+ * budget (population buffers, geometry, residual snapshot) and resident
+ * limit allow, advance together in chunks of `chunk` steps, and retire
+ * when their residual |u - u_prev|_2 / |u|_2 (over one chunk) is below tol
+ * or after max_steps (rounded up to whole chunks).  The final velocity (3
+ * planes of N floats) is copied to out_u (or a scratch buffer) before
+ * t_done_s is taken.  status: 0 converged, 1 step limit, 2 rejected (does
+ * not fit the budget), 3 diverged. */
+typedef struct lbm_request {
+  lbm_config cfg;       /* standard mode, storage = the server's */
+  const uint8_t* solid; /* one byte per node, or NULL */
+  double arrival_s;
+  int32_t max_steps;
+  double tol;
+  float* out_u;         /* 3*N floats, or NULL */
+  double t_admit_s, t_done_s; /* outputs, seconds after the start */
+  int32_t steps;
+  double residual;
+  int32_t status;
+} lbm_request;
+
+typedef struct lbm_serve_params {
+  int32_t storage;
+  int64_t budget_bytes;
+  int32_t chunk;        /* even */
+  int32_t max_resident; /* 0 = no limit */
+  int32_t autotune;     /* pick the launch shape on a scratch batch first */
+} lbm_serve_params;
+
+typedef struct lbm_serve_stats {
+  double wall_s;            /* first admission to last result */
+  int64_t lattice_updates;  /* sum over chunks of resident nodes x chunk */
+  double mlups;             /* lattice_updates / wall_s / 1e6 */
+  double p50_latency_s, p99_latency_s, mean_latency_s;
+  int32_t completed, rejected, max_resident;
+  int64_t peak_bytes;
+} lbm_serve_stats;
+
+int lbm_serve_run(const lbm_serve_params* params, lbm_request* requests,
+                  int32_t count, lbm_serve_stats* stats);
 
 /* ------------------------------------------------------------ measurement */
 /* No counterpart in the reference: copies a `bytes`-sized device buffer

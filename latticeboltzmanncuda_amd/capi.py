@@ -73,6 +73,44 @@ POINT_DTYPE = np.dtype({
     "itemsize": 144,
 })
 
+class Request(C.Structure):
+    _fields_ = [
+        ("cfg", Config),
+        ("solid", C.c_void_p),
+        ("arrival_s", C.c_double),
+        ("max_steps", C.c_int32),
+        ("tol", C.c_double),
+        ("out_u", C.c_void_p),
+        ("t_admit_s", C.c_double), ("t_done_s", C.c_double),
+        ("steps", C.c_int32),
+        ("residual", C.c_double),
+        ("status", C.c_int32),
+    ]
+
+
+class ServeParams(C.Structure):
+    _fields_ = [
+        ("storage", C.c_int32),
+        ("budget_bytes", C.c_int64),
+        ("chunk", C.c_int32),
+        ("max_resident", C.c_int32),
+        ("autotune", C.c_int32),
+    ]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [
+        ("wall_s", C.c_double),
+        ("lattice_updates", C.c_int64),
+        ("mlups", C.c_double),
+        ("p50_latency_s", C.c_double), ("p99_latency_s", C.c_double),
+        ("mean_latency_s", C.c_double),
+        ("completed", C.c_int32), ("rejected", C.c_int32),
+        ("max_resident", C.c_int32),
+        ("peak_bytes", C.c_int64),
+    ]
+
+
 P = C.c_void_p
 CTX = C.c_void_p
 i32 = C.c_int32
@@ -112,6 +150,7 @@ _SIGNATURES = {
     "lbm_set_force": (C.c_int, [CTX, P]),
     "lbm_get_force": (C.c_int, [CTX, P]),
     "lbm_step": (C.c_int, [CTX, C.c_int]),
+    "lbm_step_each": (C.c_int, [P, i32, C.c_int]),
     "lbm_step_reference": (C.c_int, [CTX, C.c_int, C.c_int]),
     "lbm_time_steps": (C.c_int, [CTX, C.c_int, C.POINTER(C.c_float)]),
     "lbm_time_steps_reference": (C.c_int, [CTX, C.c_int, C.c_int,
@@ -119,6 +158,28 @@ _SIGNATURES = {
     "lbm_moments": (C.c_int, [CTX, P, P]),
     "lbm_conservation": (C.c_int, [CTX, P]),
     "lbm_test_coordinates": (C.c_int, [CTX, C.POINTER(C.c_int)]),
+    "lbm_batch_create": (C.c_int, [C.c_int, i64, P, C.POINTER(P)]),
+    "lbm_batch_destroy": (C.c_int, [P]),
+    "lbm_batch_estimate_bytes": (i64, [C.POINTER(Config), C.c_int]),
+    "lbm_batch_add": (C.c_int, [P, C.POINTER(Config), P, C.c_int,
+                                C.POINTER(i32)]),
+    "lbm_batch_remove": (C.c_int, [P, i32]),
+    "lbm_batch_member": (C.c_int, [P, i32, C.POINTER(CTX)]),
+    "lbm_batch_info": (C.c_int, [P, C.POINTER(i32), C.POINTER(i64),
+                                 C.POINTER(i64)]),
+    "lbm_batch_set_launch": (C.c_int, [P, C.c_int, C.c_int]),
+    "lbm_batch_get_launch": (C.c_int, [P, C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int)]),
+    "lbm_batch_autotune": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]),
+    "lbm_batch_step": (C.c_int, [P, C.c_int]),
+    "lbm_batch_step_graph": (C.c_int, [P, C.c_int, C.c_int]),
+    "lbm_batch_synchronize": (C.c_int, [P]),
+    "lbm_batch_time_steps": (C.c_int, [P, C.c_int, C.c_int,
+                                       C.POINTER(C.c_float)]),
+    "lbm_batch_residuals": (C.c_int, [P, P, i32]),
+    "lbm_serve_run": (C.c_int, [C.POINTER(ServeParams), C.POINTER(Request),
+                                i32, C.POINTER(ServeStats)]),
     "lbm_measure_copy_bandwidth": (C.c_int, [i64, C.c_int,
                                              C.POINTER(C.c_double)]),
     "lbm_random_spheres": (i64, [i32, i32, i32, C.c_uint64, C.c_double,
@@ -361,6 +422,176 @@ class Context:
 
     def device_bytes(self) -> int:
         return int(self.lib.lbm_device_bytes(self.h))
+
+
+class Member(Context):
+    """A lattice owned by a Batch: the per-lattice calls of Context work on
+    it; it is freed by the batch, not by close()."""
+
+    def __init__(self, batch, slot, handle, cfg):
+        self.lib = load()
+        self.cfg = cfg
+        self.h = handle
+        self.q = cfg.lattice
+        self.dims = (cfg.nx, cfg.ny, cfg.nz)
+        self.n = cfg.nx * cfg.ny * cfg.nz
+        self.batch = batch
+        self.slot = slot
+
+    def close(self):
+        self.h = None
+
+
+class Batch:
+    """Resident lattices stepped together by one persistent kernel."""
+
+    def __init__(self, storage=STORE_F32, budget_bytes=200 * 10 ** 9,
+                 stream=None):
+        self.lib = load()
+        h = P()
+        check(self.lib.lbm_batch_create(storage, budget_bytes, stream,
+                                        C.byref(h)), "lbm_batch_create")
+        self.h = h
+        self.storage = storage
+        self.members: dict[int, Member] = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lbm_batch_destroy(self.h)
+            self.h = None
+            self.members.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @staticmethod
+    def estimate_bytes(cfg: Config, track_residual=False) -> int:
+        return int(load().lbm_batch_estimate_bytes(C.byref(cfg),
+                                                   int(track_residual)))
+
+    def add(self, cfg: Config, solid=None, track_residual=False) -> Member:
+        s = None if solid is None else np.ascontiguousarray(solid, np.uint8)
+        slot = i32(-1)
+        check(self.lib.lbm_batch_add(self.h, C.byref(cfg), _ptr(s),
+                                     int(track_residual), C.byref(slot)),
+              "lbm_batch_add")
+        ctx = CTX()
+        check(self.lib.lbm_batch_member(self.h, slot.value, C.byref(ctx)),
+              "lbm_batch_member")
+        m = Member(self, slot.value, ctx, cfg)
+        self.members[slot.value] = m
+        return m
+
+    def remove(self, member: Member):
+        check(self.lib.lbm_batch_remove(self.h, member.slot),
+              "lbm_batch_remove")
+        self.members.pop(member.slot, None)
+        member.h = None
+
+    def info(self):
+        l, n, b = i32(0), i64(0), i64(0)
+        check(self.lib.lbm_batch_info(self.h, C.byref(l), C.byref(n),
+                                      C.byref(b)), "lbm_batch_info")
+        return l.value, n.value, b.value
+
+    def set_launch(self, block, grid):
+        check(self.lib.lbm_batch_set_launch(self.h, block, grid),
+              "lbm_batch_set_launch")
+
+    def launch(self):
+        b, g = C.c_int(0), C.c_int(0)
+        check(self.lib.lbm_batch_get_launch(self.h, C.byref(b), C.byref(g)),
+              "lbm_batch_get_launch")
+        return b.value, g.value
+
+    def autotune(self, steps=10, use_cache=True):
+        b, g = C.c_int(0), C.c_int(0)
+        check(self.lib.lbm_batch_autotune(self.h, steps, int(use_cache),
+                                          C.byref(b), C.byref(g)),
+              "lbm_batch_autotune")
+        return b.value, g.value
+
+    def step(self, n=1, chunk=0):
+        if chunk:
+            check(self.lib.lbm_batch_step_graph(self.h, n, chunk),
+                  "lbm_batch_step_graph")
+        else:
+            check(self.lib.lbm_batch_step(self.h, n), "lbm_batch_step")
+
+    def synchronize(self):
+        check(self.lib.lbm_batch_synchronize(self.h), "lbm_batch_synchronize")
+
+    def time_steps(self, n, chunk=0) -> float:
+        ms = C.c_float(0)
+        check(self.lib.lbm_batch_time_steps(self.h, n, chunk, C.byref(ms)),
+              "lbm_batch_time_steps")
+        return ms.value
+
+    def residuals(self) -> dict:
+        nslots = max(self.members) + 1 if self.members else 1
+        out = np.full(nslots, -1.0)
+        check(self.lib.lbm_batch_residuals(self.h, _ptr(out), nslots),
+              "lbm_batch_residuals")
+        return {s: float(out[s]) for s in self.members}
+
+
+def step_each(contexts, n: int) -> None:
+    """n steps of every context, one single-lattice launch each per step."""
+    arr = (CTX * len(contexts))(*[c.h for c in contexts])
+    check(load().lbm_step_each(arr, len(contexts), n), "lbm_step_each")
+
+
+def serve(requests, storage=STORE_F16, budget_bytes=200 * 10 ** 9,
+          chunk=100, max_resident=0, autotune=True, keep_results=False):
+    """Runs lbm_serve_run on a list of dicts with keys cfg (Config), solid
+    (array or None), arrival_s, max_steps, tol.  Returns (stats dict, list of
+    per-request result dicts)."""
+    n = len(requests)
+    arr = (Request * n)()
+    keep = []
+    for k, r in enumerate(requests):
+        q = arr[k]
+        q.cfg = r["cfg"]
+        s = r.get("solid")
+        if s is not None:
+            s = np.ascontiguousarray(s, np.uint8)
+            keep.append(s)
+            q.solid = s.ctypes.data
+        q.arrival_s = r["arrival_s"]
+        q.max_steps = r["max_steps"]
+        q.tol = r["tol"]
+        if keep_results:
+            nn = r["cfg"].nx * r["cfg"].ny * r["cfg"].nz
+            u = np.zeros(3 * nn, np.float32)
+            keep.append(u)
+            r["_u"] = u
+            q.out_u = u.ctypes.data
+    p = ServeParams()
+    p.storage, p.budget_bytes, p.chunk = storage, budget_bytes, chunk
+    p.max_resident, p.autotune = max_resident, int(autotune)
+    st = ServeStats()
+    check(load().lbm_serve_run(C.byref(p), arr, n, C.byref(st)),
+          "lbm_serve_run")
+    stats = {f[0]: getattr(st, f[0]) for f in ServeStats._fields_}
+    results = []
+    for k, r in enumerate(requests):
+        q = arr[k]
+        res = {"t_admit_s": q.t_admit_s, "t_done_s": q.t_done_s,
+               "steps": q.steps, "residual": q.residual, "status": q.status,
+               "latency_s": q.t_done_s - q.arrival_s}
+        if keep_results:
+            res["u"] = r.pop("_u").reshape(3, -1)
+        results.append(res)
+    return stats, results
 
 
 def measure_copy_bandwidth(nbytes: int, iters: int = 20) -> float:

@@ -34,6 +34,30 @@ int hip_fail(hipError_t e, const char* what) {
   return e == hipErrorOutOfMemory ? LBM_ERR_NOMEM : LBM_ERR_HIP;
 }
 
+StdParams make_std_params(const lbm_config& cfg) {
+  StdParams p{};
+  p.tau = cfg.tau;
+  for (int a = 0; a < 3; ++a) {
+    p.force[a] = cfg.force[a];
+    p.lid_u[a] = cfg.lid_u[a];
+  }
+  p.lid_axis = cfg.lid_face >= 0 ? cfg.lid_face / 2 : -1;
+  p.lid_sign = cfg.lid_face >= 0 ? (cfg.lid_face % 2 ? 1 : -1) : 0;
+  p.wall_rho = cfg.wall_rho;
+  return p;
+}
+
+int64_t estimate_ctx_bytes(const lbm_config& cfg) {
+  const uint64_t n = (uint64_t)cfg.nx * cfg.ny * cfg.nz;
+  int64_t b = 2 * (int64_t)(population_elems(cfg.lattice, n) * storage_bytes(cfg.storage));
+  if (cfg.mode == LBM_MODE_STANDARD)
+    b += 4 * (int64_t)n + 4 * (int64_t)((n + 15) / 16);
+  else
+    b += (int64_t)n;
+  b += 4 * 1024 * 8 * 4 + 64;  // reduction scratch, force register
+  return b;
+}
+
 namespace {
 
 // Scoped device allocation for staging buffers.
@@ -106,19 +130,6 @@ int h2d(lbm_ctx* c, void* dst, const void* src, size_t bytes) {
 int d2h(lbm_ctx* c, void* dst, const void* src, size_t bytes) {
   LBM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
   return sync(c);
-}
-
-StdParams std_params(const lbm_ctx* c) {
-  StdParams p{};
-  p.tau = c->cfg.tau;
-  for (int a = 0; a < 3; ++a) {
-    p.force[a] = c->cfg.force[a];
-    p.lid_u[a] = c->cfg.lid_u[a];
-  }
-  p.lid_axis = c->cfg.lid_face >= 0 ? c->cfg.lid_face / 2 : -1;
-  p.lid_sign = c->cfg.lid_face >= 0 ? (c->cfg.lid_face % 2 ? 1 : -1) : 0;
-  p.wall_rho = c->cfg.wall_rho;
-  return p;
 }
 
 int validate_config(const lbm_config* cfg) {
@@ -205,7 +216,7 @@ int rebuild_special(lbm_ctx* c) {
 }
 
 int step_standard(lbm_ctx* c, int n) {
-  const StdParams p = std_params(c);
+  const StdParams p = make_std_params(c->cfg);
   for (int s = 0; s < n; ++s) {
     int rc = launch_std_step(c->q, c->cfg.storage, c->grid, c->f[c->cur],
                              c->f[c->cur ^ 1], c->mask, c->cls, p,
@@ -647,6 +658,19 @@ int lbm_step(lbm_ctx* c, int n) {
   if (n < 0) return fail(LBM_ERR_INVALID, "negative step count");
   return c->cfg.mode == LBM_MODE_STANDARD ? step_standard(c, n)
                                           : step_reference_fused(c, n);
+}
+
+int lbm_step_each(lbm_ctx* const* ctxs, int32_t count, int n) {
+  if (!ctxs || count < 0 || n < 0) return fail(LBM_ERR_INVALID, "bad arguments");
+  for (int32_t k = 0; k < count; ++k) {
+    if (int rc = check_ctx(ctxs[k])) return rc;
+    if (ctxs[k]->cfg.mode != LBM_MODE_STANDARD)
+      return fail(LBM_ERR_STATE, "lbm_step_each needs standard-mode lattices");
+  }
+  for (int s = 0; s < n; ++s)
+    for (int32_t k = 0; k < count; ++k)
+      if (int rc = step_standard(ctxs[k], 1)) return rc;
+  return LBM_OK;
 }
 
 int lbm_step_reference(lbm_ctx* c, int n, int with_force) {

@@ -59,6 +59,10 @@ struct FlowSpec {
   const uint8_t* mask;
 };
 
+StdParams make_std_params(const lbm_config& cfg);
+// device bytes lbm_create allocates for cfg (populations, geometry, scratch)
+int64_t estimate_ctx_bytes(const lbm_config& cfg);
+
 // error helpers --------------------------------------------------------
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);

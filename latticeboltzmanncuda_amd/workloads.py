@@ -1,0 +1,198 @@
+"""Synthetic workloads of the benchmark configurations, built on the C-ABI.
+
+cavity          lid-driven cavity (D3Q19 n^3 or D2Q9 n^2), lid = y max face
+channel         D3Q19 body-force channel through random spheres (periodic x)
+taylor_green    z-extruded Taylor-Green vortex on a periodic box
+precision_study FP16/BF16/FP32 shifted-storage runs against an FP64 run
+
+All populations live on the GPU; only setup arrays and readbacks cross the
+bus.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import capi
+
+
+def cavity(n: int, lattice: int = capi.D3Q19, tau: float = 0.6,
+           lid_u: float = 0.1, storage: int = capi.STORE_F32
+           ) -> capi.Context:
+    nz = 1 if lattice == capi.D2Q9 else n
+    ctx = capi.Context(capi.make_config(n, n, nz, lattice=lattice,
+                                        storage=storage, tau=tau,
+                                        lid_face=3, lid_u=(lid_u, 0, 0)))
+    ctx.init_equilibrium()
+    return ctx
+
+
+def channel(dims=(256, 128, 128), tau=0.8, force=1e-6, seed=2,
+            solid_fraction=0.2, storage=capi.STORE_F32):
+    solid, spheres = capi.random_spheres(dims, seed, solid_fraction, 4.0,
+                                         8.0, True)
+    ctx = capi.Context(capi.make_config(*dims, lattice=capi.D3Q19,
+                                        storage=storage, tau=tau,
+                                        periodic=(1, 0, 0),
+                                        force=(force, 0, 0)))
+    ctx.upload_solid(solid)
+    ctx.init_equilibrium()
+    return ctx, solid, spheres
+
+
+def taylor_green_fields(L: int, nz: int, u0: float):
+    """rho (N) and u (3 x N) of the z-extruded vortex
+    u = u0 sin(kx) cos(ky), v = -u0 cos(kx) sin(ky), w = 0, k = 2 pi / L,
+    rho = 1 - (3 u0^2 / 4)(cos 2kx + cos 2ky)."""
+    k = 2.0 * math.pi / L
+    x = np.arange(L, dtype=np.float64)
+    sx, cx = np.sin(k * x), np.cos(k * x)
+    c2x = np.cos(2 * k * x)
+    n = L * L * nz
+    ux = (u0 * sx[:, None] * cx[None, :])[:, :, None]
+    uy = (-u0 * cx[:, None] * sx[None, :])[:, :, None]
+    rho = (1.0 - 0.75 * u0 * u0 * (c2x[:, None] + c2x[None, :]))[:, :, None]
+    shape = (L, L, nz)
+    u = np.empty((3, n), np.float64)
+    u[0] = np.broadcast_to(ux, shape).reshape(-1)
+    u[1] = np.broadcast_to(uy, shape).reshape(-1)
+    u[2] = 0.0
+    return np.broadcast_to(rho, shape).reshape(-1).copy(), u
+
+
+def taylor_green(L: int = 256, nz: int | None = None, u0: float = 0.05,
+                 tau: float = 0.55, lattice: int = capi.D3Q19,
+                 storage: int = capi.STORE_F32) -> capi.Context:
+    nz = L if nz is None else nz
+    if lattice == capi.D2Q9:
+        nz = 1
+    ctx = capi.Context(capi.make_config(L, L, nz, lattice=lattice,
+                                        storage=storage, tau=tau,
+                                        periodic=(1, 1, 1)))
+    rho, u = taylor_green_fields(L, nz, u0)
+    ctx.init_equilibrium(rho, u)
+    return ctx
+
+
+def kinetic_energy(rho: np.ndarray, u: np.ndarray) -> float:
+    return 0.5 * float(np.sum(rho.astype(np.float64) *
+                              (u.astype(np.float64) ** 2).sum(0)))
+
+
+def precision_study(L: int = 256, steps: int = 2000, u0: float = 0.05,
+                    tau: float = 0.55,
+                    storages=("f16", "bf16", "f32")) -> dict:
+    """Runs the Taylor-Green vortex in FP64 storage (the reference run) and
+    in each shifted 16-bit / FP32 storage, on the GPU, and reports velocity
+    errors against the FP64 run, kinetic-energy decay against FP64 and the
+    analytic rate exp(-4 nu k^2 t), and MLUPS of each run."""
+    nu = (tau - 0.5) / 3.0
+    k = 2.0 * math.pi / L
+    analytic = math.exp(-4.0 * nu * k * k * steps)
+    n = L * L * L
+    out = {"workload": f"D3Q19 Taylor-Green {L}^3 periodic, u0={u0}, "
+                       f"tau={tau}, {steps} steps",
+           "analytic_ke_ratio": analytic, "runs": {}}
+
+    def run(storage_name):
+        st = capi.STORAGE_NAMES[storage_name]
+        ctx = taylor_green(L, L, u0, tau, capi.D3Q19, st)
+        rho0, u_init = ctx.moments()
+        ke0 = kinetic_energy(rho0, u_init)
+        ms = ctx.time_steps(steps)
+        rho, u = ctx.moments()
+        ctx.close()
+        return rho, u, ke0, ms
+
+    rho64, u64, ke0_64, ms64 = run("f64")
+    ke64 = kinetic_energy(rho64, u64)
+    norm2 = float(np.sqrt((u64.astype(np.float64) ** 2).sum()))
+    umax = float(np.abs(u64).max())
+    out["runs"]["f64"] = {
+        "mlups": n * steps / (ms64 / 1e3) / 1e6,
+        "ke_ratio": ke64 / ke0_64,
+        "ke_ratio_vs_analytic": ke64 / ke0_64 / analytic - 1.0,
+    }
+    for name in storages:
+        rho, u, ke0, ms = run(name)
+        d = u.astype(np.float64) - u64.astype(np.float64)
+        ke = kinetic_energy(rho, u)
+        out["runs"][name] = {
+            "mlups": n * steps / (ms / 1e3) / 1e6,
+            "u_l2_rel_err": float(np.sqrt((d ** 2).sum())) / norm2,
+            "u_linf_rel_err": float(np.abs(d).max()) / umax,
+            "ke_ratio": ke / ke0,
+            "ke_rel_err_vs_f64": (ke / ke0) / (ke64 / ke0_64) - 1.0,
+            "ke_ratio_vs_analytic": ke / ke0 / analytic - 1.0,
+        }
+        del rho, u, d
+    return out
+
+
+# --------------------------------------------------------------- batching
+def random_squares(nx: int, ny: int, frac: float, rng) -> np.ndarray:
+    """Solid mask (nx*ny bytes, index y + ny*x) of random axis-aligned
+    squares, side U{2..max(2, min(nx, ny)//8)}, added until at least frac of
+    the nodes are solid."""
+    solid = np.zeros((nx, ny), np.uint8)
+    smax = max(2, min(nx, ny) // 8)
+    target = frac * nx * ny
+    while solid.sum() < target:
+        s = int(rng.integers(2, smax + 1))
+        x0 = int(rng.integers(0, max(1, nx - s + 1)))
+        y0 = int(rng.integers(0, max(1, ny - s + 1)))
+        solid[x0:x0 + s, y0:y0 + s] = 1
+    return solid.reshape(-1)
+
+
+def c4_specs(count: int = 512, seed: int = 4, smin: int = 32, smax: int = 512,
+             U: float = 0.1, tau_min: float = 0.51, solid_frac: float = 0.1,
+             storage: int = capi.STORE_F32):
+    """The batching workload: count D2Q9 lattices, sides log-uniform in
+    [smin, smax] per axis, half lid-driven cavities (lid = y max face at
+    U) and half body-force channels (periodic x, walls y, force
+    8 nu U / ny^2 for a centre-line speed U), Reynolds number U[100, 1000],
+    tau = 1/2 + 3 U L / Re with L the lid length (cavity) or the height
+    (channel), clamped to tau >= tau_min, random squares to solid_frac.
+    Returns a list of (config, solid, info) and the number of clamped
+    lattices."""
+    rng = np.random.default_rng(seed)
+    out, clamped = [], 0
+    for _ in range(count):
+        nx = int(round(math.exp(rng.uniform(math.log(smin), math.log(smax)))))
+        ny = int(round(math.exp(rng.uniform(math.log(smin), math.log(smax)))))
+        kind = "cavity" if rng.random() < 0.5 else "channel"
+        re = float(rng.uniform(100.0, 1000.0))
+        L = nx if kind == "cavity" else ny
+        tau = 0.5 + 3.0 * U * L / re
+        if tau < tau_min:
+            tau = tau_min
+            clamped += 1
+        solid = random_squares(nx, ny, solid_frac, rng)
+        if kind == "cavity":
+            cfg = capi.make_config(nx, ny, 1, lattice=capi.D2Q9,
+                                   storage=storage, tau=tau, lid_face=3,
+                                   lid_u=(U, 0, 0))
+        else:
+            nu = (tau - 0.5) / 3.0
+            cfg = capi.make_config(nx, ny, 1, lattice=capi.D2Q9,
+                                   storage=storage, tau=tau,
+                                   periodic=(1, 0, 0),
+                                   force=(8.0 * nu * U / (ny * ny), 0, 0))
+        out.append((cfg, solid, {"kind": kind, "re": re, "tau": tau,
+                                 "nx": nx, "ny": ny}))
+    return out, clamped
+
+
+def with_storage(cfg: capi.Config, storage: int) -> capi.Config:
+    c = capi.Config()
+    C_fields = [f[0] for f in capi.Config._fields_]
+    for name in C_fields:
+        v = getattr(cfg, name)
+        if hasattr(v, "__len__"):
+            getattr(c, name)[:] = list(v)
+        else:
+            setattr(c, name, v)
+    c.storage = storage
+    return c

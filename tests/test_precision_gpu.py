@@ -1,0 +1,57 @@
+"""C3 precision: the Taylor-Green vortex with FP16 / BF16 shifted storage
+(f_i - w_i in 16 bits, FP32 collide) and FP32 storage against an FP64 GPU
+run, at reduced size (full size runs in bench.py)."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from latticeboltzmanncuda_amd import capi, workloads
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_taylor_green_precision_64():
+    say("case: TGV 64^3 u0=0.05 tau=0.55 x 500, f64/f32/f16/bf16")
+    res = workloads.precision_study(L=64, steps=500, u0=0.05, tau=0.55)
+    for k, v in res["runs"].items():
+        say(f"  {k}: {v}")
+    r = res["runs"]
+    # the FP64 run follows the analytic decay (BGK error O(k^2))
+    assert abs(r["f64"]["ke_ratio_vs_analytic"]) < 2e-3
+    # FP32 storage is indistinguishable from FP64 at this scale
+    assert r["f32"]["u_l2_rel_err"] < 1e-5
+    # FP16 shifted storage: <= 1e-3 relative velocity error vs FP64
+    assert r["f16"]["u_l2_rel_err"] < 1e-3
+    assert r["f16"]["u_linf_rel_err"] < 2e-3
+    assert abs(r["f16"]["ke_rel_err_vs_f64"]) < 1e-3
+    # BF16 (8-bit mantissa) is an order of magnitude coarser; it must still
+    # track the flow
+    assert r["bf16"]["u_l2_rel_err"] < 5e-2
+
+
+def test_fp64_gpu_matches_fp64_oracle_tgv():
+    # the FP64 GPU run is the reference of the precision study: pin it to
+    # the CPU oracle's FP64 run on a small vortex
+    say("case: TGV FP64 GPU vs FP64 oracle 24x24x4 x 100")
+    L, nz, u0, tau, steps = 24, 4, 0.05, 0.55, 100
+    rho, u = workloads.taylor_green_fields(L, nz, u0)
+    with capi.Context(capi.make_config(L, L, nz, lattice=19,
+                                       storage=capi.STORE_F64, tau=tau,
+                                       periodic=(1, 1, 1))) as c:
+        c.init_equilibrium(rho, u)
+        c.step(steps)
+        fg = c.populations(np.float64)
+    dims = (L, L, nz)
+    mask = O.std_build_mask(19, dims, None, (1, 1, 1))
+    p = O.std_params(tau=tau, periodic=(1, 1, 1))
+    fo = O.std_run_f64(19, dims, O.std_init_eq(19, dims, rho, u), mask, p,
+                       steps)
+    assert np.abs(fg - fo).max() < 1e-13

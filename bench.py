@@ -5,7 +5,12 @@ walls in y/z, spheres of radius U[4,8] from MT19937-64(seed 2) up to 20 %
 solid, Guo force 1e-6 along x, rho = 1, u = 0.
 
 A step is one fused pull collide+stream launch over the whole lattice with
-the populations already resident in HBM.  Prints one JSON line.
+the populations already resident in HBM.  Prints one JSON line.  Besides the
+headline it carries three sections (skipped with --headline-only):
+precision (Taylor-Green 256^3 in FP16/BF16/FP32 storage vs FP64),
+batched (512 D2Q9 lattices, per-lattice launches vs the persistent batch
+kernel, FP32 and FP16) and serving (Poisson requests through the
+continuous-batching server, batched vs per-lattice engine).
 
     python bench.py [--gpus 1] [--steps 5000] [--warmup 100]
 """
@@ -41,7 +46,94 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="wall time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="skip the precision, batched and serving sections")
+    ap.add_argument("--c3-steps", type=int, default=2000)
+    ap.add_argument("--c4-steps", type=int, default=500)
+    ap.add_argument("--c5-requests", type=int, default=64)
+    ap.add_argument("--c5-lambda", type=float, default=100.0)
+    ap.add_argument("--c5-max-steps", type=int, default=20000)
     return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def precision_section(steps):
+    """Taylor-Green 256^3, FP16 / BF16 / FP32 shifted storage against the
+    FP64 GPU run and the analytic decay."""
+    from latticeboltzmanncuda_amd import workloads
+    res = workloads.precision_study(L=256, steps=steps, u0=0.05, tau=0.55)
+    runs = {k: {kk: (round(vv, 9) if isinstance(vv, float) else vv)
+                for kk, vv in v.items()} for k, v in res["runs"].items()}
+    return {"workload": res["workload"],
+            "analytic_ke_ratio": round(res["analytic_ke_ratio"], 9),
+            "runs": runs}
+
+
+def batched_section(steps):
+    """512 D2Q9 cavity/channel lattices (seed 4): plain per-lattice
+    launches vs one persistent launch per step in hipGraph chunks of 100,
+    FP32 and FP16 storage; FP16 velocity error against FP64 storage."""
+    from latticeboltzmanncuda_amd import capi, workloads
+    out = {"workload": "512 D2Q9 lattices (seed 4), sides log-uniform "
+                       "32-512, cavity/channel, Re 100-1000, tau >= 0.51, "
+                       "10% squares", "steps": steps, "chunk": 100}
+    specs32, clamped = workloads.c4_specs(512, storage=capi.STORE_F32)
+    out["tau_clamped"] = clamped
+    out["nodes"] = int(sum(c.nx * c.ny for c, _, _ in specs32))
+    for name, st, bpl in (("f32", capi.STORE_F32, 72),
+                          ("f16", capi.STORE_F16, 36)):
+        specs = [(workloads.with_storage(c, st), s, i)
+                 for c, s, i in specs32]
+        log(f"batched: {name} plain")
+        plain = workloads.c4_plain_mlups(specs, steps)
+        log(f"batched: {name} persistent")
+        bat = workloads.c4_batched_mlups(specs, steps, 100, True, st)
+        out[name] = {
+            "plain_mlups": round(plain["mlups"], 1),
+            "batched_mlups": round(bat["mlups"], 1),
+            "speedup": round(bat["mlups"] / plain["mlups"], 3),
+            "batched_hbm_frac": round(bat["mlups"] * 1e6 * bpl / 8e12, 4),
+            "plain_hbm_frac": round(plain["mlups"] * 1e6 * bpl / 8e12, 4),
+            "launch": {"block": bat["block"], "grid": bat["grid"]},
+        }
+    log("batched: f16 vs f64 velocity error")
+    err = workloads.c4_velocity_error(specs32, capi.STORE_F16, 1000)
+    out["f16"]["u_l2_rel_err_vs_f64"] = round(err["u_l2_rel_err"], 8)
+    out["f16"]["worst_lattice_rel_err"] = round(err["worst_lattice"], 8)
+    out["f16"]["error_steps"] = err["steps"]
+    return out
+
+
+def serving_section(nreq, lam, max_steps):
+    """Continuous-batching solve server, FP16 storage, Poisson arrivals:
+    batched engine vs per-lattice launches on the same requests."""
+    from latticeboltzmanncuda_amd import capi, workloads
+    reqs = workloads.c5_requests(lam, nreq, max_steps=max_steps)
+    out = {"workload": f"{nreq} requests (seed 5), Poisson lambda {lam}/s, "
+                       "50% D2Q9 64-1024^2 squares, 50% D3Q19 32-128^3 "
+                       "spheres, 5-25% solid, tau 0.55-1.0, body-force "
+                       f"channel, residual 1e-6 or {max_steps} steps, "
+                       "200 GB budget, FP16 storage, chunk 100"}
+    for name, eng in (("batched", capi.ENGINE_BATCHED),
+                      ("per_lattice", capi.ENGINE_PER_LATTICE)):
+        log(f"serving: {name}")
+        st, res = capi.serve(reqs, storage=capi.STORE_F16, chunk=100,
+                             engine=eng)
+        conv = sum(1 for r in res if r["status"] == 0)
+        out[name] = {
+            "mlups": round(st["mlups"], 1),
+            "p50_latency_s": round(st["p50_latency_s"], 4),
+            "p99_latency_s": round(st["p99_latency_s"], 4),
+            "mean_latency_s": round(st["mean_latency_s"], 4),
+            "wall_s": round(st["wall_s"], 3),
+            "completed": st["completed"], "converged": conv,
+            "rejected": st["rejected"], "max_resident": st["max_resident"],
+            "peak_gb": round(st["peak_bytes"] / 1e9, 3),
+        }
+    return out
 
 
 def cpu_baseline(solid: np.ndarray, seconds: float) -> dict:
@@ -164,6 +256,14 @@ def main():
         },
     }
     ctx.close()
+    if not args.headline_only:
+        log("precision")
+        out["precision"] = precision_section(args.c3_steps)
+        log("batched")
+        out["batched"] = batched_section(args.c4_steps)
+        log("serving")
+        out["serving"] = serving_section(args.c5_requests, args.c5_lambda,
+                                         args.c5_max_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(solid, args.cpu_seconds)
     print(json.dumps(out))

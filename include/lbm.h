@@ -308,7 +308,14 @@ typedef struct lbm_serve_params {
   int32_t chunk;        /* even */
   int32_t max_resident; /* 0 = no limit */
   int32_t autotune;     /* pick the launch shape on a scratch batch first */
+  int32_t engine;       /* LBM_ENGINE_BATCHED or LBM_ENGINE_PER_LATTICE */
 } lbm_serve_params;
+
+/* how a server advances its resident lattices by one chunk */
+enum {
+  LBM_ENGINE_BATCHED = 0,     /* one persistent launch per step, hipGraph chunks */
+  LBM_ENGINE_PER_LATTICE = 1  /* one plain launch per lattice per step */
+};
 
 typedef struct lbm_serve_stats {
   double wall_s;            /* first admission to last result */

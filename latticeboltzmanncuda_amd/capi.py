@@ -22,6 +22,7 @@ OK = 0
 D2Q9, D3Q19, D3Q27 = 9, 19, 27
 STORE_F32, STORE_F64, STORE_F16, STORE_BF16 = 0, 1, 2, 3
 MODE_STANDARD, MODE_REFERENCE = 0, 1
+ENGINE_BATCHED, ENGINE_PER_LATTICE = 0, 1
 FLOW_NONE, FLOW_POINT, FLOW_PLANE, FLOW_BOX, FLOW_MASK = 0, 1, 2, 3, 4
 FLAG_INTERNAL, FLAG_REFLECTED = 1, 2
 MASK_SOLID = 1 << 31
@@ -95,6 +96,7 @@ class ServeParams(C.Structure):
         ("chunk", C.c_int32),
         ("max_resident", C.c_int32),
         ("autotune", C.c_int32),
+        ("engine", C.c_int32),
     ]
 
 
@@ -551,10 +553,13 @@ def step_each(contexts, n: int) -> None:
 
 
 def serve(requests, storage=STORE_F16, budget_bytes=200 * 10 ** 9,
-          chunk=100, max_resident=0, autotune=True, keep_results=False):
+          chunk=100, max_resident=0, autotune=True, keep_results=False,
+          engine=ENGINE_BATCHED):
     """Runs lbm_serve_run on a list of dicts with keys cfg (Config), solid
-    (array or None), arrival_s, max_steps, tol.  Returns (stats dict, list of
-    per-request result dicts)."""
+    (array or None), arrival_s, max_steps, tol.  engine: ENGINE_BATCHED
+    (persistent kernel, hipGraph chunks) or ENGINE_PER_LATTICE (one plain
+    launch per lattice per step).  Returns (stats dict, list of per-request
+    result dicts)."""
     n = len(requests)
     arr = (Request * n)()
     keep = []
@@ -578,6 +583,7 @@ def serve(requests, storage=STORE_F16, budget_bytes=200 * 10 ** 9,
     p = ServeParams()
     p.storage, p.budget_bytes, p.chunk = storage, budget_bytes, chunk
     p.max_resident, p.autotune = max_resident, int(autotune)
+    p.engine = engine
     st = ServeStats()
     check(load().lbm_serve_run(C.byref(p), arr, n, C.byref(st)),
           "lbm_serve_run")

@@ -90,6 +90,6 @@ struct lbm_batch {
   // captured chunk
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
-  int graph_chunk = 0, graph_block = 0, graph_grid = -1;
+  int graph_chunk = 0, graph_block = 0, graph_grid = -1;  // grid as launched
   void* graph_tables = nullptr;
 };

@@ -186,13 +186,13 @@ int capture_chunk(lbm_batch* b, int chunk) {
   }
   b->graph_chunk = chunk;
   b->graph_block = b->block;
-  b->graph_grid = b->grid;
+  b->graph_grid = effective_grid(b);
   return LBM_OK;
 }
 
 int ensure_graph(lbm_batch* b, int chunk) {
   if (b->graph_exec && b->graph_chunk == chunk && b->graph_block == b->block &&
-      b->graph_grid == b->grid && b->graph_tables == (void*)b->d_tiles)
+      b->graph_grid == effective_grid(b) && b->graph_tables == (void*)b->d_tiles)
     return LBM_OK;
   int rc = capture_chunk(b, chunk);
   if (rc) return rc;

@@ -8,8 +8,9 @@
 //      population buffers, geometry, residual snapshot) and the resident
 //      limit allow -- requests that arrive while others run join at the
 //      next chunk boundary as memory frees up;
-//   3. advances every resident lattice by one chunk with the batch's
-//      persistent kernel (replayed hipGraph);
+//   3. advances every resident lattice by one chunk: with the batch's
+//      persistent kernel (replayed hipGraph), or -- the comparison
+//      engine -- with one plain launch per lattice per step;
 //   4. measures all residuals in one launch, retires finished lattices
 //      (velocity copied to the host, latency = arrival -> result on host)
 //      and frees their memory.
@@ -83,6 +84,8 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
     return fail(LBM_ERR_INVALID, "null argument");
   if (p->chunk < 2 || (p->chunk & 1))
     return fail(LBM_ERR_INVALID, "chunk must be an even step count >= 2");
+  if (p->engine != LBM_ENGINE_BATCHED && p->engine != LBM_ENGINE_PER_LATTICE)
+    return fail(LBM_ERR_INVALID, "unknown serving engine");
   for (int32_t k = 0; k < nreq; ++k) {
     if (int rc = validate_request(reqs[k], p->storage)) return rc;
     reqs[k].t_admit_s = reqs[k].t_done_s = -1.0;
@@ -94,7 +97,7 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
   lbm_batch* b = nullptr;
   int rc = lbm_batch_create(p->storage, p->budget_bytes, nullptr, &b);
   if (rc) return rc;
-  if (p->autotune && (rc = pretune(*p, reqs, nreq, b))) {
+  if (p->engine == LBM_ENGINE_BATCHED && p->autotune && (rc = pretune(*p, reqs, nreq, b))) {
     lbm_batch_destroy(b);
     return rc;
   }
@@ -108,6 +111,7 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
   int32_t resident = 0, next = 0, finished = 0;
   std::vector<double> residuals;
   std::vector<float> scratch;
+  std::vector<lbm_ctx*> members;
   int64_t updates = 0;
   const auto t0 = Clock::now();
   double t_first = -1.0, t_last = 0.0;
@@ -153,7 +157,17 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
       if (p->max_resident > 0 && resident >= p->max_resident) break;
       int32_t slot = -1;
       const int e = lbm_batch_add(b, &reqs[r].cfg, reqs[r].solid, 1, &slot);
-      if (e == LBM_ERR_BUDGET) break;  // wait for memory to free up
+      if (e == LBM_ERR_BUDGET && resident > 0) break;  // wait for memory to free up
+      if (e == LBM_ERR_BUDGET) {
+        // nothing resident to wait for: the request with the batch tables
+        // does not fit the budget
+        reqs[r].status = 2;
+        reqs[r].t_done_s = seconds_since(t0);
+        fifo.pop_front();
+        ++finished;
+        ++stats->rejected;
+        continue;
+      }
       if (e) {
         lbm_batch_destroy(b);
         return e;
@@ -180,7 +194,17 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
     // one chunk for everybody resident
     int64_t nodes = 0;
     lbm_batch_info(b, nullptr, &nodes, nullptr);
-    if ((rc = lbm_batch_step_graph(b, p->chunk, p->chunk))) {
+    if (p->engine == LBM_ENGINE_PER_LATTICE) {
+      members.clear();
+      for (int32_t s = 0; s < (int32_t)slot_req.size(); ++s) {
+        lbm_ctx* c = nullptr;
+        if (slot_req[s] >= 0 && lbm_batch_member(b, s, &c) == LBM_OK) members.push_back(c);
+      }
+      rc = lbm_step_each(members.data(), (int32_t)members.size(), p->chunk);
+    } else {
+      rc = lbm_batch_step_graph(b, p->chunk, p->chunk);
+    }
+    if (rc) {
       lbm_batch_destroy(b);
       return rc;
     }

@@ -196,3 +196,123 @@ def with_storage(cfg: capi.Config, storage: int) -> capi.Config:
             setattr(c, name, v)
     c.storage = storage
     return c
+
+
+def c4_plain_mlups(specs, steps: int) -> dict:
+    """Every lattice in its own context (own stream), one plain launch per
+    lattice per step; wall-clock MLUPS over all lattices."""
+    import time
+    ctxs = []
+    try:
+        for cfg, solid, _ in specs:
+            c = capi.Context(cfg)
+            c.upload_solid(solid)
+            c.init_equilibrium()
+            ctxs.append(c)
+        nodes = sum(c.cfg.nx * c.cfg.ny * c.cfg.nz for c in ctxs)
+        capi.step_each(ctxs, 2)
+        for c in ctxs:
+            c.synchronize()
+        t0 = time.perf_counter()
+        capi.step_each(ctxs, steps)
+        for c in ctxs:
+            c.synchronize()
+        dt = time.perf_counter() - t0
+    finally:
+        for c in ctxs:
+            c.close()
+    return {"mlups": nodes * steps / dt / 1e6, "ms_per_step": dt * 1e3 / steps,
+            "nodes": nodes, "steps": steps}
+
+
+def c4_batched_mlups(specs, steps: int, chunk: int = 100,
+                     autotune: bool = True, storage: int = capi.STORE_F32
+                     ) -> dict:
+    """All lattices in one batch: one persistent launch per step, replayed
+    as hipGraph chunks of `chunk` steps, launch shape from the autotuner;
+    wall-clock MLUPS (hipEvents agree within launch jitter)."""
+    import time
+    with capi.Batch(storage) as b:
+        for cfg, solid, _ in specs:
+            b.add(cfg, solid)
+        _, nodes, nbytes = b.info()
+        block, grid = b.autotune(steps=10) if autotune else b.launch()
+        b.step(chunk, chunk)  # capture + warm-up
+        b.synchronize()
+        t0 = time.perf_counter()
+        ms = b.time_steps(steps, chunk)
+        dt = time.perf_counter() - t0
+    return {"mlups": nodes * steps / dt / 1e6,
+            "mlups_events": nodes * steps / (ms / 1e3) / 1e6,
+            "ms_per_step": dt * 1e3 / steps, "nodes": nodes, "steps": steps,
+            "block": block, "grid": grid, "device_bytes": nbytes}
+
+
+def c4_velocity_error(specs, storage: int, steps: int, chunk: int = 100
+                      ) -> dict:
+    """Velocity of every lattice after `steps` batched steps in `storage`
+    against the same batch in FP64 storage: aggregate relative L2 error
+    (sum over lattices of |u - u64|^2 over sum |u64|^2) and the worst
+    single lattice."""
+    def run(st):
+        out = []
+        with capi.Batch(st) as b:
+            members = [b.add(with_storage(cfg, st), solid)
+                       for cfg, solid, _ in specs]
+            b.set_launch(256, 0)
+            b.step(steps, chunk)
+            b.synchronize()
+            for m in members:
+                out.append(m.moments()[1].astype(np.float64))
+        return out
+
+    u64 = run(capi.STORE_F64)
+    ux = run(storage)
+    num = den = 0.0
+    worst = 0.0
+    for a, b in zip(ux, u64):
+        d2 = float(((a - b) ** 2).sum())
+        n2 = float((b ** 2).sum())
+        num += d2
+        den += n2
+        worst = max(worst, math.sqrt(d2 / n2) if n2 > 0 else 0.0)
+    return {"u_l2_rel_err": math.sqrt(num / den), "worst_lattice": worst,
+            "steps": steps}
+
+
+def c5_requests(lam: float, count: int, seed: int = 5,
+                storage: int = capi.STORE_F16, max_steps: int = 20000,
+                tol: float = 1e-6, u_target: float = 0.05) -> list:
+    """The serving workload: `count` requests with Poisson arrivals at rate
+    lam per second.  Half D2Q9 squares n^2 (n log-uniform 64..1024, random
+    squares), half D3Q19 cubes n^3 (n log-uniform 32..128, random spheres of
+    radius n/16..n/8, at least 2); solid fraction U[0.05, 0.25]; tau
+    U[0.55, 1.0]; body-force channel (periodic x, bounce-back walls) with
+    force 8 nu u_target / ny^2; run to residual tol or max_steps."""
+    rng = np.random.default_rng(seed)
+    out, t = [], 0.0
+    for _ in range(count):
+        t += float(rng.exponential(1.0 / lam))
+        frac = float(rng.uniform(0.05, 0.25))
+        tau = float(rng.uniform(0.55, 1.0))
+        nu = (tau - 0.5) / 3.0
+        if rng.random() < 0.5:
+            n = int(round(math.exp(rng.uniform(math.log(64), math.log(1024)))))
+            solid = random_squares(n, n, frac, rng)
+            cfg = capi.make_config(n, n, 1, lattice=capi.D2Q9,
+                                   storage=storage, tau=tau,
+                                   periodic=(1, 0, 0),
+                                   force=(8.0 * nu * u_target / (n * n), 0, 0))
+        else:
+            n = int(round(math.exp(rng.uniform(math.log(32), math.log(128)))))
+            rmin = max(2.0, n / 16.0)
+            rmax = max(rmin + 1.0, n / 8.0)
+            solid, _ = capi.random_spheres((n, n, n), int(rng.integers(1 << 31)),
+                                           frac, rmin, rmax, True)
+            cfg = capi.make_config(n, n, n, lattice=capi.D3Q19,
+                                   storage=storage, tau=tau,
+                                   periodic=(1, 0, 0),
+                                   force=(8.0 * nu * u_target / (n * n), 0, 0))
+        out.append({"cfg": cfg, "solid": solid, "arrival_s": t,
+                    "max_steps": max_steps, "tol": tol})
+    return out

@@ -14,7 +14,7 @@ namespace {
 
 // Workgroups walk the tile table with a grid stride; every tile belongs to
 // one lattice, so the descriptor lookup is uniform across the workgroup.
-template <int S, int BS>
+template <int S, int BS, bool NT>
 __global__ void __launch_bounds__(BS)
     k_batch_step(const BatchLattice* __restrict__ lats,
                  const BatchTile* __restrict__ tiles,
@@ -31,13 +31,13 @@ __global__ void __launch_bounds__(BS)
       T* B = (T*)L.f[cur ^ 1];
       switch (L.q) {
         case 9:
-          std_node_step<9, S, true>(L.g, A, B, L.mask, L.cls, L.p, n);
+          std_node_step<9, S, true, NT>(L.g, A, B, L.mask, L.cls, L.p, n);
           break;
         case 19:
-          std_node_step<19, S, true>(L.g, A, B, L.mask, L.cls, L.p, n);
+          std_node_step<19, S, true, NT>(L.g, A, B, L.mask, L.cls, L.p, n);
           break;
         default:
-          std_node_step<27, S, true>(L.g, A, B, L.mask, L.cls, L.p, n);
+          std_node_step<27, S, true, NT>(L.g, A, B, L.mask, L.cls, L.p, n);
           break;
       }
     }
@@ -124,23 +124,23 @@ __global__ void __launch_bounds__(kResidualChunk)
   }
 }
 
-template <int S>
+template <int S, bool NT>
 int launch_step_s(int block, int grid, const BatchLattice* lats,
                   const BatchTile* tiles, const uint32_t* ntiles, int parity,
                   hipStream_t s) {
   const dim3 g((unsigned)grid);
   switch (block) {
     case 64:
-      hipLaunchKernelGGL((k_batch_step<S, 64>), g, dim3(64), 0, s, lats, tiles, ntiles, parity);
+      hipLaunchKernelGGL((k_batch_step<S, 64, NT>), g, dim3(64), 0, s, lats, tiles, ntiles, parity);
       break;
     case 128:
-      hipLaunchKernelGGL((k_batch_step<S, 128>), g, dim3(128), 0, s, lats, tiles, ntiles, parity);
+      hipLaunchKernelGGL((k_batch_step<S, 128, NT>), g, dim3(128), 0, s, lats, tiles, ntiles, parity);
       break;
     case 256:
-      hipLaunchKernelGGL((k_batch_step<S, 256>), g, dim3(256), 0, s, lats, tiles, ntiles, parity);
+      hipLaunchKernelGGL((k_batch_step<S, 256, NT>), g, dim3(256), 0, s, lats, tiles, ntiles, parity);
       break;
     case 512:
-      hipLaunchKernelGGL((k_batch_step<S, 512>), g, dim3(512), 0, s, lats, tiles, ntiles, parity);
+      hipLaunchKernelGGL((k_batch_step<S, 512, NT>), g, dim3(512), 0, s, lats, tiles, ntiles, parity);
       break;
     default:
       return fail(LBM_ERR_INVALID, "batch block size must be 64, 128, 256 or 512");
@@ -149,21 +149,29 @@ int launch_step_s(int block, int grid, const BatchLattice* lats,
   return LBM_OK;
 }
 
+template <int S>
+int launch_step_nt(bool nt, int block, int grid, const BatchLattice* lats,
+                   const BatchTile* tiles, const uint32_t* ntiles, int parity,
+                   hipStream_t s) {
+  return nt ? launch_step_s<S, true>(block, grid, lats, tiles, ntiles, parity, s)
+            : launch_step_s<S, false>(block, grid, lats, tiles, ntiles, parity, s);
+}
+
 }  // namespace
 
-int launch_batch_step(int storage, int block, int grid,
+int launch_batch_step(int storage, bool nt, int block, int grid,
                       const BatchLattice* lats, const BatchTile* tiles,
                       const uint32_t* ntiles, int parity, hipStream_t s) {
   if (grid <= 0) return fail(LBM_ERR_INVALID, "batch grid must be positive");
   switch (storage) {
     case LBM_STORE_F32:
-      return launch_step_s<LBM_STORE_F32>(block, grid, lats, tiles, ntiles, parity, s);
+      return launch_step_nt<LBM_STORE_F32>(nt, block, grid, lats, tiles, ntiles, parity, s);
     case LBM_STORE_F64:
-      return launch_step_s<LBM_STORE_F64>(block, grid, lats, tiles, ntiles, parity, s);
+      return launch_step_nt<LBM_STORE_F64>(nt, block, grid, lats, tiles, ntiles, parity, s);
     case LBM_STORE_F16:
-      return launch_step_s<LBM_STORE_F16>(block, grid, lats, tiles, ntiles, parity, s);
+      return launch_step_nt<LBM_STORE_F16>(nt, block, grid, lats, tiles, ntiles, parity, s);
     case LBM_STORE_BF16:
-      return launch_step_s<LBM_STORE_BF16>(block, grid, lats, tiles, ntiles, parity, s);
+      return launch_step_nt<LBM_STORE_BF16>(nt, block, grid, lats, tiles, ntiles, parity, s);
     default:
       return fail(LBM_ERR_UNSUPPORTED, "unknown storage");
   }

@@ -6,9 +6,10 @@
 // One thread per node over a flat 1-D grid (coalesced along n, whatever the
 // lattice shape); (x, y, z) come from two magic-number divisions.
 //
-// The per-node update is std_node_step (lbm_step_node.h).
-// Consecutive steps walk the tiles in opposite orders so that the start of
-// a step reads what the previous step wrote last.
+// The per-node update is std_node_step (lbm_step_node.h).  Lattices whose
+// two population copies exceed the Infinity Cache use non-temporal stores
+// and walk the tiles in opposite orders on consecutive steps; smaller ones
+// use ordinary stores and a fixed order (both measured fastest on MI355X).
 #include <hip/hip_runtime.h>
 
 #include "lbm_internal.h"
@@ -30,7 +31,7 @@ __device__ __forceinline__ void decode(const Grid& g, uint32_t n, uint32_t& x,
   y = t - x * g.ny;
 }
 
-template <int Q, int S, bool LID>
+template <int Q, int S, bool LID, bool NT>
 __global__ void __launch_bounds__(kBlock)
     k_std_step(Grid g, const typename Store<S>::T* __restrict__ A,
                typename Store<S>::T* __restrict__ B,
@@ -39,7 +40,7 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t blk = rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
   const uint32_t n = blk * kBlock + threadIdx.x;
   if (n >= g.n) return;
-  std_node_step<Q, S, LID>(g, A, B, mask, cls, p, n);
+  std_node_step<Q, S, LID, NT>(g, A, B, mask, cls, p, n);
 }
 
 template <int Q>
@@ -275,17 +276,28 @@ int launch_std_step(int q, int storage, const Grid& g, const void* A, void* B,
                     const StdParams& p, int rev, hipStream_t s) {
   const unsigned nb = blocks_for(g.n);
   const bool lid = p.lid_axis >= 0;
+  const bool nt = streaming_stores(2 * population_elems(q, g.n) * storage_bytes(storage));
+  // measured on MI355X: with streaming stores, alternating the tile order
+  // between steps is fastest; without them, a fixed order is
+  const int r = nt ? rev : 0;
+  const dim3 grid(nb), block(kBlock);
   const int drc = dispatch_qs(q, storage, [&](auto tag) {
     using Tag = decltype(tag);
     using T = typename Store<Tag::S>::T;
-    if (lid)
-      hipLaunchKernelGGL((k_std_step<Tag::Q, Tag::S, true>), dim3(nb),
-                         dim3(kBlock), 0, s, g, (const T*)A, (T*)B, mask, cls,
-                         p, rev);
+    const T* a = (const T*)A;
+    T* b = (T*)B;
+    if (lid && nt)
+      hipLaunchKernelGGL((k_std_step<Tag::Q, Tag::S, true, true>), grid, block, 0, s, g, a, b,
+                         mask, cls, p, r);
+    else if (lid)
+      hipLaunchKernelGGL((k_std_step<Tag::Q, Tag::S, true, false>), grid, block, 0, s, g, a, b,
+                         mask, cls, p, r);
+    else if (nt)
+      hipLaunchKernelGGL((k_std_step<Tag::Q, Tag::S, false, true>), grid, block, 0, s, g, a, b,
+                         mask, cls, p, r);
     else
-      hipLaunchKernelGGL((k_std_step<Tag::Q, Tag::S, false>), dim3(nb),
-                         dim3(kBlock), 0, s, g, (const T*)A, (T*)B, mask, cls,
-                         p, rev);
+      hipLaunchKernelGGL((k_std_step<Tag::Q, Tag::S, false, false>), grid, block, 0, s, g, a, b,
+                         mask, cls, p, r);
   });
   if (drc) return drc;
   LBM_HIP(hipGetLastError());

@@ -43,7 +43,7 @@ struct BatchTile {
 // does not depend on the launch shape or on the other members.
 constexpr uint32_t kResidualChunk = 256;
 
-int launch_batch_step(int storage, int block, int grid,
+int launch_batch_step(int storage, bool nt, int block, int grid,
                       const BatchLattice* lats, const BatchTile* tiles,
                       const uint32_t* ntiles, int parity, hipStream_t s);
 int launch_batch_residual(int storage, const BatchLattice* lats,
@@ -86,10 +86,12 @@ struct lbm_batch {
   int block = 256;
   int grid = 0;  // 0: one workgroup per tile (not persistent)
   int tables_block = 0;  // block size the tile table was built for
+  bool nt = false;       // non-temporal stores: members exceed the Infinity Cache
 
   // captured chunk
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_chunk = 0, graph_block = 0, graph_grid = -1;  // grid as launched
+  bool graph_nt = false;
   void* graph_tables = nullptr;
 };

@@ -73,6 +73,7 @@ bool tables_stale(const lbm_batch* b) {
 int sync_tables(lbm_batch* b) {
   if (!tables_stale(b)) return LBM_OK;
   std::vector<BatchTile> tiles, chunks;
+  uint64_t pop_bytes = 0;
   b->h_lats.clear();
   b->h_lat_slot.clear();
   b->h_res_count.clear();
@@ -90,6 +91,7 @@ int sync_tables(lbm_batch* b) {
     d.off = c->cur;
     d.u_prev = b->snapshots[s];
     d.res_first = (uint32_t)chunks.size();
+    pop_bytes += 2 * population_elems(c->q, c->grid.n) * storage_bytes(b->storage);
     const uint32_t idx = (uint32_t)b->h_lats.size();
     for (uint32_t first = 0; first < c->grid.n; first += (uint32_t)b->block)
       tiles.push_back({idx, first});
@@ -124,6 +126,7 @@ int sync_tables(lbm_batch* b) {
     hipError_t e = hipMalloc((void**)&b->d_ntiles, sizeof(uint32_t));
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(batch tile count)");
   }
+  b->nt = streaming_stores(pop_bytes);
   b->ntiles = (uint32_t)tiles.size();
   b->nchunks = (uint32_t)chunks.size();
   if (!b->h_lats.empty())
@@ -151,8 +154,8 @@ int effective_grid(const lbm_batch* b) {
 int launch_steps(lbm_batch* b, int n) {
   const int grid = effective_grid(b);
   for (int k = 0; k < n; ++k) {
-    int rc = launch_batch_step(b->storage, b->block, grid, b->d_lats, b->d_tiles, b->d_ntiles,
-                               k & 1, b->stream);
+    int rc = launch_batch_step(b->storage, b->nt, b->block, grid, b->d_lats, b->d_tiles,
+                               b->d_ntiles, k & 1, b->stream);
     if (rc) return rc;
   }
   return LBM_OK;
@@ -187,12 +190,14 @@ int capture_chunk(lbm_batch* b, int chunk) {
   b->graph_chunk = chunk;
   b->graph_block = b->block;
   b->graph_grid = effective_grid(b);
+  b->graph_nt = b->nt;
   return LBM_OK;
 }
 
 int ensure_graph(lbm_batch* b, int chunk) {
   if (b->graph_exec && b->graph_chunk == chunk && b->graph_block == b->block &&
-      b->graph_grid == effective_grid(b) && b->graph_tables == (void*)b->d_tiles)
+      b->graph_grid == effective_grid(b) && b->graph_nt == b->nt &&
+      b->graph_tables == (void*)b->d_tiles)
     return LBM_OK;
   int rc = capture_chunk(b, chunk);
   if (rc) return rc;

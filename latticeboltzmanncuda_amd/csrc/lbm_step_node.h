@@ -13,7 +13,8 @@
 //   3. collide (BGK + Guo forcing) in FP32 (FP64 for F64 storage);
 //   4. store every direction with all lanes active; solid nodes store the
 //      rest state w_i.  Skipping the stores of solid lanes leaves partly
-//      written cache lines that cost ~1.7x in time on MI355X.
+//      written cache lines that cost ~1.7x in time on MI355X.  NT selects
+//      non-temporal stores (lattices larger than the Infinity Cache).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,7 +35,7 @@ __device__ __forceinline__ void decode_node(const Grid& g, uint32_t n,
   y = t - x * g.ny;
 }
 
-template <int Q, int S, bool LID>
+template <int Q, int S, bool LID, bool NT = false>
 __device__ __forceinline__ void std_node_step(
     const Grid& g, const typename Store<S>::T* __restrict__ A,
     typename Store<S>::T* __restrict__ B, const uint32_t* __restrict__ mask,
@@ -114,7 +115,10 @@ __device__ __forceinline__ void std_node_step(
     C out = f[i] + omega * (feq - f[i]) + src;
     out = solid ? w : out;
     if (St::kShifted) out -= w;
-    St::store(B + pidx<Q>(n, i), out);
+    if (NT)
+      St::store_nt(B + pidx<Q>(n, i), out);
+    else
+      St::store(B + pidx<Q>(n, i), out);
   }
 }
 

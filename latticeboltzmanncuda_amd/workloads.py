@@ -131,17 +131,20 @@ def precision_study(L: int = 256, steps: int = 2000, u0: float = 0.05,
 
 
 # --------------------------------------------------------------- batching
-def random_squares(nx: int, ny: int, frac: float, rng) -> np.ndarray:
+def random_squares(nx: int, ny: int, frac: float, rng,
+                   clear_top: int = 0) -> np.ndarray:
     """Solid mask (nx*ny bytes, index y + ny*x) of random axis-aligned
     squares, side U{2..max(2, min(nx, ny)//8)}, added until at least frac of
-    the nodes are solid."""
+    the nodes are solid.  The top clear_top rows stay fluid (a square that
+    touches a moving lid makes a corner pocket whose density the lid drains
+    until BGK diverges)."""
     solid = np.zeros((nx, ny), np.uint8)
     smax = max(2, min(nx, ny) // 8)
     target = frac * nx * ny
     while solid.sum() < target:
         s = int(rng.integers(2, smax + 1))
         x0 = int(rng.integers(0, max(1, nx - s + 1)))
-        y0 = int(rng.integers(0, max(1, ny - s + 1)))
+        y0 = int(rng.integers(0, max(1, ny - clear_top - s + 1)))
         solid[x0:x0 + s, y0:y0 + s] = 1
     return solid.reshape(-1)
 
@@ -169,7 +172,8 @@ def c4_specs(count: int = 512, seed: int = 4, smin: int = 32, smax: int = 512,
         if tau < tau_min:
             tau = tau_min
             clamped += 1
-        solid = random_squares(nx, ny, solid_frac, rng)
+        solid = random_squares(nx, ny, solid_frac, rng,
+                               clear_top=2 if kind == "cavity" else 0)
         if kind == "cavity":
             cfg = capi.make_config(nx, ny, 1, lattice=capi.D2Q9,
                                    storage=storage, tau=tau, lid_face=3,
@@ -252,8 +256,9 @@ def c4_velocity_error(specs, storage: int, steps: int, chunk: int = 100
                       ) -> dict:
     """Velocity of every lattice after `steps` batched steps in `storage`
     against the same batch in FP64 storage: aggregate relative L2 error
-    (sum over lattices of |u - u64|^2 over sum |u64|^2) and the worst
-    single lattice."""
+    (sum over lattices of |u - u64|^2 over sum |u64|^2), per-lattice
+    median / 90th percentile / worst, over the lattices whose FP64 run stays
+    finite and below twice the driving speed (the others are counted)."""
     def run(st):
         out = []
         with capi.Batch(st) as b:
@@ -269,15 +274,31 @@ def c4_velocity_error(specs, storage: int, steps: int, chunk: int = 100
     u64 = run(capi.STORE_F64)
     ux = run(storage)
     num = den = 0.0
-    worst = 0.0
-    for a, b in zip(ux, u64):
+    per, unstable, diverged = [], 0, 0
+    for (cfg, _, info), a, b in zip(specs, ux, u64):
+        # a lattice whose FP64 run is not finite or exceeds twice the
+        # driving speed has no reference solution to compare with
+        if not np.isfinite(b).all():
+            unstable += 1
+            continue
+        if float(np.abs(b).max()) > 0.2:
+            unstable += 1
+            continue
+        if not np.isfinite(a).all():
+            diverged += 1
+            continue
         d2 = float(((a - b) ** 2).sum())
         n2 = float((b ** 2).sum())
         num += d2
         den += n2
-        worst = max(worst, math.sqrt(d2 / n2) if n2 > 0 else 0.0)
-    return {"u_l2_rel_err": math.sqrt(num / den), "worst_lattice": worst,
-            "steps": steps}
+        per.append(math.sqrt(d2 / n2) if n2 > 0 else 0.0)
+    per = np.array(per) if per else np.zeros(1)
+    return {"u_l2_rel_err": math.sqrt(num / den) if den > 0 else float("nan"),
+            "median_lattice": float(np.median(per)),
+            "p90_lattice": float(np.percentile(per, 90)),
+            "worst_lattice": float(per.max()),
+            "compared": int(len(per)), "unstable_in_f64": unstable,
+            "diverged_only_in_storage": diverged, "steps": steps}
 
 
 def c5_requests(lam: float, count: int, seed: int = 5,

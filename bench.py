@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="skip the precision, batched and serving sections")
+    ap.add_argument("--sections", default="precision,batched,serving",
+                    help="comma list of the extra sections to run")
     ap.add_argument("--c3-steps", type=int, default=2000)
     ap.add_argument("--c4-steps", type=int, default=500)
     ap.add_argument("--c5-requests", type=int, default=64)
@@ -256,11 +258,14 @@ def main():
         },
     }
     ctx.close()
-    if not args.headline_only:
+    sections = set() if args.headline_only else set(args.sections.split(","))
+    if "precision" in sections:
         log("precision")
         out["precision"] = precision_section(args.c3_steps)
+    if "batched" in sections:
         log("batched")
         out["batched"] = batched_section(args.c4_steps)
+    if "serving" in sections:
         log("serving")
         out["serving"] = serving_section(args.c5_requests, args.c5_lambda,
                                          args.c5_max_steps)

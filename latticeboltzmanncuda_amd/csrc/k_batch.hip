@@ -1,5 +1,6 @@
-// Batched kernels: one persistent launch updates every resident lattice of
-// a batch (any mix of D2Q9 / D3Q19 / D3Q27 with one storage type), and a
+// Batched kernels: per step, one persistent launch per kernel class
+// (lattice type x moving wall) updates every resident lattice of that class
+// (a batch mixes D2Q9 / D3Q19 / D3Q27 with one storage type), and a
 // residual kernel measures |u - u_prev| / |u| per lattice.
 #include <hip/hip_runtime.h>
 
@@ -12,10 +13,13 @@
 namespace lbm {
 namespace {
 
-// Workgroups walk the tile table with a grid stride; every tile belongs to
-// one lattice, so the descriptor lookup is uniform across the workgroup.
-template <int S, int BS, bool NT>
-__global__ void __launch_bounds__(BS)
+// Workgroups walk their class's tile table with a grid stride; a tile is
+// blockDim.x consecutive nodes of one lattice, so the descriptor lookup is
+// uniform across the workgroup.
+constexpr int kMaxBatchBlock = 512;
+
+template <int Q, int S, bool LID, bool NT>
+__global__ void __launch_bounds__(kMaxBatchBlock)
     k_batch_step(const BatchLattice* __restrict__ lats,
                  const BatchTile* __restrict__ tiles,
                  const uint32_t* __restrict__ ntiles, int parity) {
@@ -27,19 +31,8 @@ __global__ void __launch_bounds__(BS)
     const uint32_t n = tile.first + threadIdx.x;
     if (n < L.g.n) {
       const int cur = (L.off + parity) & 1;
-      const T* A = (const T*)L.f[cur];
-      T* B = (T*)L.f[cur ^ 1];
-      switch (L.q) {
-        case 9:
-          std_node_step<9, S, true, NT>(L.g, A, B, L.mask, L.cls, L.p, n);
-          break;
-        case 19:
-          std_node_step<19, S, true, NT>(L.g, A, B, L.mask, L.cls, L.p, n);
-          break;
-        default:
-          std_node_step<27, S, true, NT>(L.g, A, B, L.mask, L.cls, L.p, n);
-          break;
-      }
+      std_node_step<Q, S, LID, NT>(L.g, (const T*)L.f[cur], (T*)L.f[cur ^ 1], L.mask, L.cls,
+                                   L.p, n);
     }
   }
 }
@@ -83,12 +76,13 @@ __device__ __forceinline__ void residual_node(const BatchLattice& L, uint32_t n,
   const double uz = (double)((jz - (C)0.5 * (C)L.p.force[2]) / r);
   float* up = L.u_prev;
   const uint64_t N = L.g.n;
-  const double dx = ux - up[n], dy = uy - up[N + n], dz = uz - up[2 * N + n];
+  const double dx = ux - gload(up + n), dy = uy - gload(up + N + n),
+               dz = uz - gload(up + 2 * N + n);
   d2 = dx * dx + dy * dy + dz * dz;
   u2 = ux * ux + uy * uy + uz * uz;
-  up[n] = (float)ux;
-  up[N + n] = (float)uy;
-  up[2 * N + n] = (float)uz;
+  gstore(up + n, (float)ux);
+  gstore(up + N + n, (float)uy);
+  gstore(up + 2 * N + n, (float)uz);
 }
 
 template <int S>
@@ -124,57 +118,38 @@ __global__ void __launch_bounds__(kResidualChunk)
   }
 }
 
-template <int S, bool NT>
-int launch_step_s(int block, int grid, const BatchLattice* lats,
-                  const BatchTile* tiles, const uint32_t* ntiles, int parity,
-                  hipStream_t s) {
-  const dim3 g((unsigned)grid);
-  switch (block) {
-    case 64:
-      hipLaunchKernelGGL((k_batch_step<S, 64, NT>), g, dim3(64), 0, s, lats, tiles, ntiles, parity);
-      break;
-    case 128:
-      hipLaunchKernelGGL((k_batch_step<S, 128, NT>), g, dim3(128), 0, s, lats, tiles, ntiles, parity);
-      break;
-    case 256:
-      hipLaunchKernelGGL((k_batch_step<S, 256, NT>), g, dim3(256), 0, s, lats, tiles, ntiles, parity);
-      break;
-    case 512:
-      hipLaunchKernelGGL((k_batch_step<S, 512, NT>), g, dim3(512), 0, s, lats, tiles, ntiles, parity);
-      break;
-    default:
-      return fail(LBM_ERR_INVALID, "batch block size must be 64, 128, 256 or 512");
-  }
+template <int Q, int S>
+int launch_step_q(bool lid, bool nt, int block, int grid, const BatchLattice* lats,
+                  const BatchTile* tiles, const uint32_t* ntiles, int parity, hipStream_t s) {
+  const dim3 g((unsigned)grid), b((unsigned)block);
+  if (lid && nt)
+    hipLaunchKernelGGL((k_batch_step<Q, S, true, true>), g, b, 0, s, lats, tiles, ntiles, parity);
+  else if (lid)
+    hipLaunchKernelGGL((k_batch_step<Q, S, true, false>), g, b, 0, s, lats, tiles, ntiles, parity);
+  else if (nt)
+    hipLaunchKernelGGL((k_batch_step<Q, S, false, true>), g, b, 0, s, lats, tiles, ntiles, parity);
+  else
+    hipLaunchKernelGGL((k_batch_step<Q, S, false, false>), g, b, 0, s, lats, tiles, ntiles, parity);
   LBM_HIP(hipGetLastError());
   return LBM_OK;
 }
 
-template <int S>
-int launch_step_nt(bool nt, int block, int grid, const BatchLattice* lats,
-                   const BatchTile* tiles, const uint32_t* ntiles, int parity,
-                   hipStream_t s) {
-  return nt ? launch_step_s<S, true>(block, grid, lats, tiles, ntiles, parity, s)
-            : launch_step_s<S, false>(block, grid, lats, tiles, ntiles, parity, s);
-}
-
 }  // namespace
 
-int launch_batch_step(int storage, bool nt, int block, int grid,
+int launch_batch_step(int storage, int cls, bool nt, int block, int grid,
                       const BatchLattice* lats, const BatchTile* tiles,
                       const uint32_t* ntiles, int parity, hipStream_t s) {
   if (grid <= 0) return fail(LBM_ERR_INVALID, "batch grid must be positive");
-  switch (storage) {
-    case LBM_STORE_F32:
-      return launch_step_nt<LBM_STORE_F32>(nt, block, grid, lats, tiles, ntiles, parity, s);
-    case LBM_STORE_F64:
-      return launch_step_nt<LBM_STORE_F64>(nt, block, grid, lats, tiles, ntiles, parity, s);
-    case LBM_STORE_F16:
-      return launch_step_nt<LBM_STORE_F16>(nt, block, grid, lats, tiles, ntiles, parity, s);
-    case LBM_STORE_BF16:
-      return launch_step_nt<LBM_STORE_BF16>(nt, block, grid, lats, tiles, ntiles, parity, s);
-    default:
-      return fail(LBM_ERR_UNSUPPORTED, "unknown storage");
-  }
+  if (block != 64 && block != 128 && block != 256 && block != kMaxBatchBlock)
+    return fail(LBM_ERR_INVALID, "batch block size must be 64, 128, 256 or 512");
+  if (cls < 0 || cls >= kBatchClasses) return fail(LBM_ERR_INVALID, "bad kernel class");
+  const bool lid = class_lid(cls);
+  int rc = LBM_OK;
+  const int drc = dispatch_qs(class_q(cls), storage, [&](auto tag) {
+    using Tag = decltype(tag);
+    rc = launch_step_q<Tag::Q, Tag::S>(lid, nt, block, grid, lats, tiles, ntiles, parity, s);
+  });
+  return drc ? drc : rc;
 }
 
 int launch_batch_residual(int storage, const BatchLattice* lats,

@@ -43,7 +43,17 @@ struct BatchTile {
 // does not depend on the launch shape or on the other members.
 constexpr uint32_t kResidualChunk = 256;
 
-int launch_batch_step(int storage, bool nt, int block, int grid,
+// Tiles are kept in one table per kernel class (lattice type x moving
+// wall), and each step launches one persistent kernel per non-empty class:
+// each kernel then holds only its own lattice type's registers and code.
+constexpr int kBatchClasses = 6;
+inline int batch_class(int q, bool lid) {
+  return (q == 9 ? 0 : (q == 19 ? 1 : 2)) * 2 + (lid ? 1 : 0);
+}
+inline int class_q(int c) { return c < 2 ? 9 : (c < 4 ? 19 : 27); }
+inline bool class_lid(int c) { return (c & 1) != 0; }
+
+int launch_batch_step(int storage, int cls, bool nt, int block, int grid,
                       const BatchLattice* lats, const BatchTile* tiles,
                       const uint32_t* ntiles, int parity, hipStream_t s);
 int launch_batch_residual(int storage, const BatchLattice* lats,
@@ -68,9 +78,9 @@ struct lbm_batch {
   // device tables (capacities in entries)
   lbm::BatchLattice* d_lats = nullptr;
   uint32_t lats_cap = 0;
-  lbm::BatchTile* d_tiles = nullptr;
-  uint32_t tiles_cap = 0;
-  uint32_t* d_ntiles = nullptr;
+  lbm::BatchTile* d_tiles[lbm::kBatchClasses] = {};
+  uint32_t tiles_cap[lbm::kBatchClasses] = {};
+  uint32_t* d_ntiles = nullptr;  // tiles per class (device copy)
   lbm::BatchTile* d_chunks = nullptr;  // residual chunks
   uint32_t chunks_cap = 0;
   double* d_partials = nullptr;        // 2 per residual chunk
@@ -79,6 +89,8 @@ struct lbm_batch {
   std::vector<lbm::BatchLattice> h_lats;
   std::vector<int32_t> h_lat_slot;     // descriptor index -> slot
   std::vector<uint32_t> h_res_count;   // residual chunks per descriptor
+  std::vector<uint64_t> h_param_version;  // member parameter versions
+  uint32_t ntiles_c[lbm::kBatchClasses] = {};
   uint32_t ntiles = 0, nchunks = 0;
   bool dirty = true;
 
@@ -91,7 +103,10 @@ struct lbm_batch {
   // captured chunk
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
-  int graph_chunk = 0, graph_block = 0, graph_grid = -1;  // grid as launched
+  int graph_chunk = 0, graph_block = 0;
   bool graph_nt = false;
-  void* graph_tables = nullptr;
+  // per class: tile table and grid the captured launches use (null / 0 for
+  // a class that was empty)
+  void* graph_tables[lbm::kBatchClasses] = {};
+  int graph_grids[lbm::kBatchClasses] = {};
 };

@@ -30,6 +30,8 @@ int64_t member_bytes(const lbm_config& cfg, int track) {
 }
 
 void drop_graph(lbm_batch* b) {
+  // replays of the executable graph may still be running on the stream
+  if (b->graph_exec && b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->graph_exec = nullptr;
@@ -63,7 +65,7 @@ bool tables_stale(const lbm_batch* b) {
   if (b->dirty || b->tables_block != b->block) return true;
   for (size_t k = 0; k < b->h_lats.size(); ++k) {
     const lbm_ctx* c = b->slots[b->h_lat_slot[k]];
-    if (b->h_lats[k].off != c->cur) return true;
+    if (b->h_lats[k].off != c->cur || b->h_param_version[k] != c->params_version) return true;
   }
   return false;
 }
@@ -72,11 +74,12 @@ bool tables_stale(const lbm_batch* b) {
 // set (in slot order) and copies them to the device on the batch stream.
 int sync_tables(lbm_batch* b) {
   if (!tables_stale(b)) return LBM_OK;
-  std::vector<BatchTile> tiles, chunks;
+  std::vector<BatchTile> tiles[kBatchClasses], chunks;
   uint64_t pop_bytes = 0;
   b->h_lats.clear();
   b->h_lat_slot.clear();
   b->h_res_count.clear();
+  b->h_param_version.clear();
   for (size_t s = 0; s < b->slots.size(); ++s) {
     lbm_ctx* c = b->slots[s];
     if (!c) continue;
@@ -93,20 +96,25 @@ int sync_tables(lbm_batch* b) {
     d.res_first = (uint32_t)chunks.size();
     pop_bytes += 2 * population_elems(c->q, c->grid.n) * storage_bytes(b->storage);
     const uint32_t idx = (uint32_t)b->h_lats.size();
+    const int cls = batch_class(c->q, d.p.lid_axis >= 0);
     for (uint32_t first = 0; first < c->grid.n; first += (uint32_t)b->block)
-      tiles.push_back({idx, first});
+      tiles[cls].push_back({idx, first});
     uint32_t nres = 0;
     for (uint32_t first = 0; first < c->grid.n; first += kResidualChunk, ++nres)
       chunks.push_back({idx, first});
     b->h_lats.push_back(d);
     b->h_lat_slot.push_back((int32_t)s);
     b->h_res_count.push_back(nres);
+    b->h_param_version.push_back(c->params_version);
   }
   int rc;
   if ((rc = grow(b, &b->d_lats, &b->lats_cap, (uint32_t)b->h_lats.size(), "hipMalloc(batch lattices)")) ||
-      (rc = grow(b, &b->d_tiles, &b->tiles_cap, (uint32_t)tiles.size(), "hipMalloc(batch tiles)")) ||
       (rc = grow(b, &b->d_chunks, &b->chunks_cap, (uint32_t)chunks.size(), "hipMalloc(batch chunks)")))
     return rc;
+  for (int k = 0; k < kBatchClasses; ++k)
+    if (!tiles[k].empty() &&
+        (rc = grow(b, &b->d_tiles[k], &b->tiles_cap[k], (uint32_t)tiles[k].size(), "hipMalloc(batch tiles)")))
+      return rc;
   if (!b->d_partials || b->partials_cap < b->chunks_cap) {
     if (b->d_partials) {
       (void)hipStreamSynchronize(b->stream);
@@ -123,23 +131,29 @@ int sync_tables(lbm_batch* b) {
     b->bytes += (int64_t)sizeof(double) * 2 * b->partials_cap;
   }
   if (!b->d_ntiles) {
-    hipError_t e = hipMalloc((void**)&b->d_ntiles, sizeof(uint32_t));
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc(batch tile count)");
+    hipError_t e = hipMalloc((void**)&b->d_ntiles, sizeof(uint32_t) * kBatchClasses);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(batch tile counts)");
+    b->bytes += sizeof(uint32_t) * kBatchClasses;
   }
   b->nt = streaming_stores(pop_bytes);
-  b->ntiles = (uint32_t)tiles.size();
+  b->ntiles = 0;
+  for (int k = 0; k < kBatchClasses; ++k) {
+    b->ntiles_c[k] = (uint32_t)tiles[k].size();
+    b->ntiles += b->ntiles_c[k];
+  }
   b->nchunks = (uint32_t)chunks.size();
   if (!b->h_lats.empty())
     LBM_HIP(hipMemcpyAsync(b->d_lats, b->h_lats.data(), sizeof(BatchLattice) * b->h_lats.size(),
                            hipMemcpyHostToDevice, b->stream));
-  if (!tiles.empty())
-    LBM_HIP(hipMemcpyAsync(b->d_tiles, tiles.data(), sizeof(BatchTile) * tiles.size(),
-                           hipMemcpyHostToDevice, b->stream));
+  for (int k = 0; k < kBatchClasses; ++k)
+    if (!tiles[k].empty())
+      LBM_HIP(hipMemcpyAsync(b->d_tiles[k], tiles[k].data(), sizeof(BatchTile) * tiles[k].size(),
+                             hipMemcpyHostToDevice, b->stream));
   if (!chunks.empty())
     LBM_HIP(hipMemcpyAsync(b->d_chunks, chunks.data(), sizeof(BatchTile) * chunks.size(),
                            hipMemcpyHostToDevice, b->stream));
-  LBM_HIP(hipMemcpyAsync(b->d_ntiles, &b->ntiles, sizeof(uint32_t), hipMemcpyHostToDevice,
-                         b->stream));
+  LBM_HIP(hipMemcpyAsync(b->d_ntiles, b->ntiles_c, sizeof(uint32_t) * kBatchClasses,
+                         hipMemcpyHostToDevice, b->stream));
   // the host vectors die here: wait for the copies
   LBM_HIP(hipStreamSynchronize(b->stream));
   b->dirty = false;
@@ -147,17 +161,25 @@ int sync_tables(lbm_batch* b) {
   return LBM_OK;
 }
 
-int effective_grid(const lbm_batch* b) {
-  return b->grid > 0 ? b->grid : (int)std::max<uint32_t>(b->ntiles, 1);
+// Workgroups launched for class k: the configured persistent grid (fixed,
+// so a captured graph survives admissions and evictions; surplus
+// workgroups find no tile and exit), or one workgroup per tile when grid is
+// 0; 0 for an empty class, which is not launched.
+int class_grid(const lbm_batch* b, int k) {
+  const uint32_t t = b->ntiles_c[k];
+  if (t == 0) return 0;
+  return b->grid > 0 ? b->grid : (int)t;
 }
 
 int launch_steps(lbm_batch* b, int n) {
-  const int grid = effective_grid(b);
-  for (int k = 0; k < n; ++k) {
-    int rc = launch_batch_step(b->storage, b->nt, b->block, grid, b->d_lats, b->d_tiles,
-                               b->d_ntiles, k & 1, b->stream);
-    if (rc) return rc;
-  }
+  for (int s = 0; s < n; ++s)
+    for (int k = 0; k < kBatchClasses; ++k) {
+      const int grid = class_grid(b, k);
+      if (grid == 0) continue;
+      int rc = launch_batch_step(b->storage, k, b->nt, b->block, grid, b->d_lats, b->d_tiles[k],
+                                 b->d_ntiles + k, s & 1, b->stream);
+      if (rc) return rc;
+    }
   return LBM_OK;
 }
 
@@ -189,20 +211,31 @@ int capture_chunk(lbm_batch* b, int chunk) {
   }
   b->graph_chunk = chunk;
   b->graph_block = b->block;
-  b->graph_grid = effective_grid(b);
   b->graph_nt = b->nt;
+  for (int k = 0; k < kBatchClasses; ++k) {
+    b->graph_grids[k] = class_grid(b, k);
+    b->graph_tables[k] = b->graph_grids[k] ? (void*)b->d_tiles[k] : nullptr;
+  }
   return LBM_OK;
 }
 
+// The captured launches stay valid while every class launches the same
+// grid from the same tile table (the tables' contents may change).
+bool graph_current(const lbm_batch* b, int chunk) {
+  if (!b->graph_exec || b->graph_chunk != chunk || b->graph_block != b->block ||
+      b->graph_nt != b->nt)
+    return false;
+  for (int k = 0; k < kBatchClasses; ++k) {
+    const int grid = class_grid(b, k);
+    if (b->graph_grids[k] != grid) return false;
+    if (grid && b->graph_tables[k] != (void*)b->d_tiles[k]) return false;
+  }
+  return true;
+}
+
 int ensure_graph(lbm_batch* b, int chunk) {
-  if (b->graph_exec && b->graph_chunk == chunk && b->graph_block == b->block &&
-      b->graph_grid == effective_grid(b) && b->graph_nt == b->nt &&
-      b->graph_tables == (void*)b->d_tiles)
-    return LBM_OK;
-  int rc = capture_chunk(b, chunk);
-  if (rc) return rc;
-  b->graph_tables = (void*)b->d_tiles;
-  return LBM_OK;
+  if (graph_current(b, chunk)) return LBM_OK;
+  return capture_chunk(b, chunk);
 }
 
 // Launch-shape cache of the autotuner: (storage, lattice-mix bits, log2
@@ -272,9 +305,10 @@ int lbm_batch_destroy(lbm_batch* b) {
     if (b->slots[s]) lbm_destroy(b->slots[s]);
     if (b->snapshots[s]) (void)hipFree(b->snapshots[s]);
   }
-  for (void* p : {(void*)b->d_lats, (void*)b->d_tiles, (void*)b->d_ntiles,
-                  (void*)b->d_chunks, (void*)b->d_partials})
+  for (void* p : {(void*)b->d_lats, (void*)b->d_ntiles, (void*)b->d_chunks, (void*)b->d_partials})
     if (p) (void)hipFree(p);
+  for (int k = 0; k < kBatchClasses; ++k)
+    if (b->d_tiles[k]) (void)hipFree(b->d_tiles[k]);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "lbm_internal.h"
+#include "lbm_lattice.h"
 #include "lbm_layout.h"
 #include "lbm_store.h"
 
@@ -34,6 +35,44 @@ int hip_fail(hipError_t e, const char* what) {
   return e == hipErrorOutOfMemory ? LBM_ERR_NOMEM : LBM_ERR_HIP;
 }
 
+namespace {
+
+// Fills the per-direction constants of lattice L in precision C.  The
+// products are formed in C from the configured float values, as the
+// kernels formed them before they were hoisted.
+template <class L, class C>
+void fill_coef(const StdParams& p, StepCoef<C>& k) {
+  const C omega = (C)1 / (C)p.tau;
+  const C guo = (C)1 - (C)0.5 * omega;
+  k.omega = omega;
+  for (int a = 0; a < 3; ++a) {
+    k.force[a] = (C)p.force[a];
+    k.half_force[a] = (C)0.5 * (C)p.force[a];
+  }
+  for (int i = 0; i < L::Q; ++i) {
+    const int* e = L::E[i];
+    const C w = (C)L::W[i];
+    const C eF = (C)e[0] * k.force[0] + (C)e[1] * k.force[1] + (C)e[2] * k.force[2];
+    k.gw[i] = guo * w;
+    k.gf3[i] = k.gw[i] * ((C)3 * eF);
+    k.gf9[i] = k.gw[i] * ((C)9 * eF);
+    k.lid[i] = (C)0;
+    if (p.lid_axis >= 0 && e[p.lid_axis] == -p.lid_sign) {
+      // a link of direction i whose source node lies beyond the moving face
+      const C eu = (C)e[0] * (C)p.lid_u[0] + (C)e[1] * (C)p.lid_u[1] + (C)e[2] * (C)p.lid_u[2];
+      k.lid[i] = (C)6 * w * (C)p.wall_rho * eu;
+    }
+  }
+}
+
+template <class L>
+void fill_coefs(StdParams& p) {
+  fill_coef<L, float>(p, p.cf);
+  fill_coef<L, double>(p, p.cd);
+}
+
+}  // namespace
+
 StdParams make_std_params(const lbm_config& cfg) {
   StdParams p{};
   p.tau = cfg.tau;
@@ -44,6 +83,17 @@ StdParams make_std_params(const lbm_config& cfg) {
   p.lid_axis = cfg.lid_face >= 0 ? cfg.lid_face / 2 : -1;
   p.lid_sign = cfg.lid_face >= 0 ? (cfg.lid_face % 2 ? 1 : -1) : 0;
   p.wall_rho = cfg.wall_rho;
+  switch (cfg.lattice) {
+    case 9:
+      fill_coefs<Lat<9>>(p);
+      break;
+    case 19:
+      fill_coefs<Lat<19>>(p);
+      break;
+    default:
+      fill_coefs<Lat<27>>(p);
+      break;
+  }
   return p;
 }
 
@@ -136,6 +186,9 @@ int validate_config(const lbm_config* cfg) {
   if (!cfg) return fail(LBM_ERR_INVALID, "null config");
   if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1)
     return fail(LBM_ERR_INVALID, "lattice extents must be >= 1");
+  // the kernels multiply coordinates and extents with 24-bit multiplies
+  if (cfg->nx >= (1 << 24) || cfg->ny >= (1 << 24) || cfg->nz >= (1 << 24))
+    return fail(LBM_ERR_UNSUPPORTED, "lattice extents must be below 2^24");
   const uint64_t n = (uint64_t)cfg->nx * cfg->ny * cfg->nz;
   if (cfg->lattice != 9 && cfg->lattice != 19 && cfg->lattice != 27)
     return fail(LBM_ERR_INVALID, "lattice must be 9, 19 or 27");
@@ -395,6 +448,7 @@ int lbm_set_tau(lbm_ctx* c, float tau) {
   if (!(tau > 0.0f) || (c->cfg.mode == LBM_MODE_STANDARD && !(tau > 0.5f)))
     return fail(LBM_ERR_INVALID, "tau out of range");
   c->cfg.tau = tau;
+  ++c->params_version;
   return LBM_OK;
 }
 
@@ -402,6 +456,7 @@ int lbm_set_body_force(lbm_ctx* c, const float force[3]) {
   if (int rc = check_ctx(c)) return rc;
   if (!force) return fail(LBM_ERR_INVALID, "null force");
   for (int a = 0; a < 3; ++a) c->cfg.force[a] = force[a];
+  ++c->params_version;
   return LBM_OK;
 }
 

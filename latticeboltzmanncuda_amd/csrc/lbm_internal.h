@@ -42,6 +42,21 @@ struct Grid {
   Magic dz, dy;  // division by nz and by ny
 };
 
+// Per-lattice constants of the standard step in the compute type C,
+// precomputed on the host so that the kernels spend no vector instructions
+// on values that are the same for every node.  Index i is the direction.
+template <class C>
+struct StepCoef {
+  C omega;           // 1 / tau
+  C half_force[3];   // F / 2 (velocity correction of the Guo scheme)
+  C force[3];        // F
+  C lid[27];         // 6 w_i rho_w (e_i . u_w) if a link of direction i can
+                     // come from beyond the moving face, else 0
+  C gw[27];          // (1 - omega / 2) w_i
+  C gf3[27];         // 3 (1 - omega / 2) w_i (e_i . F)
+  C gf9[27];         // 9 (1 - omega / 2) w_i (e_i . F)
+};
+
 // Parameters of the standard (BGK) step.
 struct StdParams {
   float tau;       // relaxation time
@@ -50,6 +65,8 @@ struct StdParams {
   int lid_sign;    // +1 max face, -1 min face
   float lid_u[3];
   float wall_rho;
+  StepCoef<float> cf;   // FP32 compute (FP32 / FP16 / BF16 storage)
+  StepCoef<double> cd;  // FP64 compute (FP64 storage)
 };
 
 struct FlowSpec {
@@ -135,6 +152,7 @@ struct lbm_ctx {
   uint32_t* mask = nullptr;  // standard mode: link mask per node
   uint32_t* cls = nullptr;   // standard mode: 2-bit node classes
   uint64_t steps_done = 0;   // parity picks the tile order of the next step
+  uint64_t params_version = 0;  // bumped when tau or the body force change
   uint8_t* flags = nullptr;  // reference mode
   std::vector<uint8_t> host_flags;
 

@@ -16,15 +16,24 @@
 
 #include <cstdint>
 
+#include "lbm_global.h"
+
 namespace lbm {
 
 constexpr uint32_t kTile = 64;
 constexpr uint32_t kTileShift = 6;
 
+// Element indices are below 2^32 (lbm_create checks), so the tile number
+// n >> 6 is below 2^32 / (64 Q) < 2^24 and the product fits the GPU's
+// full-rate 24-bit multiply.
 template <int Q>
 __host__ __device__ __forceinline__ uint32_t pidx(uint32_t n, int i) {
-  return (n >> kTileShift) * (uint32_t)(Q * kTile) + (uint32_t)i * kTile +
-         (n & (kTile - 1));
+#ifdef __HIP_DEVICE_COMPILE__
+  const uint32_t tile = __umul24(n >> kTileShift, (uint32_t)(Q * kTile));
+#else
+  const uint32_t tile = (n >> kTileShift) * (uint32_t)(Q * kTile);
+#endif
+  return tile + (uint32_t)i * kTile + (n & (kTile - 1));
 }
 
 // Elements of one population array for N nodes.
@@ -39,7 +48,7 @@ constexpr uint32_t kClassSolid = 2;     // not updated; holds the rest state
 
 __device__ __forceinline__ uint32_t node_class(const uint32_t* cls,
                                                uint32_t n) {
-  return (cls[n >> 4] >> (2u * (n & 15u))) & 3u;
+  return (gload(cls + (n >> 4)) >> (2u * (n & 15u))) & 3u;
 }
 
 }  // namespace lbm

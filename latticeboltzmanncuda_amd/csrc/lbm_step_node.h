@@ -8,17 +8,26 @@
 //      nodes first read their link mask and, for each bounced link, load
 //      the half-way bounce-back value A_inv(i)(x) instead (+ the moving-wall
 //      term 6 w_i rho_w e_i.u_w when the source lies beyond the moving
-//      face) -- one load per link, the address is selected, so no solid
-//      data is fetched; solid nodes load nothing;
+//      face) -- one load per link, the address is selected without a
+//      branch, so all Q loads are in flight together; solid nodes load
+//      nothing;
 //   3. collide (BGK + Guo forcing) in FP32 (FP64 for F64 storage);
 //   4. store every direction with all lanes active; solid nodes store the
 //      rest state w_i.  Skipping the stores of solid lanes leaves partly
 //      written cache lines that cost ~1.7x in time on MI355X.  NT selects
 //      non-temporal stores (lattices larger than the Infinity Cache).
+//
+// The FP16 / BF16 paths are bound by vector-instruction issue, not by HBM,
+// so the update is written to spend few instructions per node: per-lattice
+// constants come precomputed (StepCoef), addresses need no 32-bit integer
+// multiplications, and shifted storage is collided in shifted form
+// (g_i = f_i - w_i, rho = 1 + sum g_i, j = sum e_i g_i), which also keeps
+// the small deviations instead of cancelling them against w_i.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+#include "lbm_global.h"
 #include "lbm_internal.h"
 #include "lbm_lattice.h"
 #include "lbm_layout.h"
@@ -32,7 +41,153 @@ __device__ __forceinline__ void decode_node(const Grid& g, uint32_t n,
   const uint32_t t = g.dz.div(n);
   z = n - t * g.nz;
   x = g.dy.div(t);
-  y = t - x * g.ny;
+  y = t - __umul24(x, g.ny);  // x < 2^24 and ny < 2^24 (lbm_create checks)
+}
+
+template <class C>
+__device__ __forceinline__ const StepCoef<C>& step_coef(const StdParams& p);
+template <>
+__device__ __forceinline__ const StepCoef<float>& step_coef<float>(const StdParams& p) {
+  return p.cf;
+}
+template <>
+__device__ __forceinline__ const StepCoef<double>& step_coef<double>(const StdParams& p) {
+  return p.cd;
+}
+
+// 1 / r: IEEE division in FP64; in FP32 the hardware reciprocal refined by
+// one Newton-Raphson step (error below one unit in the last place)
+__device__ __forceinline__ double recip(double r) { return 1.0 / r; }
+__device__ __forceinline__ float recip(float r) {
+  const float y = __builtin_amdgcn_rcpf(r);
+  return fmadd(fmadd(-r, y, 1.0f), y, y);
+}
+
+// Loads of one node: the Q pulled (or bounced) stored values, the link
+// mask and the node's own element index.
+template <int Q, int S>
+struct NodeLoad {
+  typename Store<S>::T raw[Q];
+  uint32_t m;    // link mask (0 for interior and solid nodes)
+  uint32_t own;  // element index of (n, direction 0)
+  bool solid;
+};
+
+// Phase 1: issue every load of node n (no value is used here, so the loads
+// of several nodes can be in flight together).
+template <int Q, int S>
+__device__ __forceinline__ void std_gather(const Grid& g,
+                                           const typename Store<S>::T* __restrict__ A,
+                                           const uint32_t* __restrict__ mask,
+                                           const uint32_t* __restrict__ cls, uint32_t n,
+                                           NodeLoad<Q, S>& ld) {
+  using L = Lat<Q>;
+  using St = Store<S>;
+  const uint32_t c = node_class(cls, n);
+  uint32_t x, y, z;
+  if (Q == 9) {  // D2Q9 lattices have nz == 1
+    z = 0;
+    x = g.dy.div(n);
+    y = n - __umul24(x, g.ny);
+  } else {
+    decode_node(g, n, x, y, z);
+  }
+  const uint32_t plane = g.ny * g.nz;
+  // offsets (mod 2^32) from n to the source node n - e for e = -1, 0, +1
+  // along each axis, with periodic wrap (a wall link is flagged in the mask
+  // and its wrapped value is replaced)
+  const uint32_t ox[3] = {x + 1 == g.nx ? plane - g.n : plane, 0u,
+                          x == 0 ? g.n - plane : 0u - plane};
+  const uint32_t oy[3] = {y + 1 == g.ny ? g.nz - plane : g.nz, 0u,
+                          y == 0 ? plane - g.nz : 0u - g.nz};
+  const uint32_t oz[3] = {z + 1 == g.nz ? 1u - g.nz : 1u, 0u,
+                          z == 0 ? g.nz - 1u : 0u - 1u};
+
+  ld.solid = c == kClassSolid;
+  ld.m = c == kClassBoundary ? gload(mask + n) : 0u;
+  ld.own = pidx<Q>(n, 0);
+  // solid lanes load nothing and keep the rest state (f = w_i, g = 0)
+#pragma unroll
+  for (int i = 0; i < Q; ++i) ld.raw[i] = St::rest(i, L::W[i]);
+  if (!ld.solid) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const uint32_t src = n + ox[L::E[i][0] + 1] + oy[L::E[i][1] + 1] + oz[L::E[i][2] + 1];
+      const uint32_t from = pidx<Q>(src, i);
+      const uint32_t back = ld.own + (uint32_t)L::INV[i] * kTile;
+      // bounced link: read the node's own opposite population instead
+      const uint32_t sel = 0u - ((ld.m >> i) & 1u);
+      ld.raw[i] = gload(elem(A, from ^ ((from ^ back) & sel)));
+    }
+  }
+}
+
+// Phase 2: collide the loaded node and store its Q post-collision values.
+template <int Q, int S, bool LID, bool NT>
+__device__ __forceinline__ void std_collide(const StdParams& p,
+                                            typename Store<S>::T* __restrict__ B,
+                                            const NodeLoad<Q, S>& ld) {
+  // a * b + c is fused only where one source expression has that form, so
+  // every instantiation (single-lattice or batch, any block size, with or
+  // without a moving wall) rounds identically
+#pragma clang fp contract(on)
+  using L = Lat<Q>;
+  using St = Store<S>;
+  using C = typename St::C;
+  constexpr bool kSh = St::kShifted;
+  const StepCoef<C>& k = step_coef<C>(p);
+  const uint32_t m = ld.m;
+  // g[i]: the stored quantity (f_i, or f_i - w_i for shifted storage), plus
+  // the moving-wall term of bounced links from beyond the moving face
+  C gv[Q];
+  const bool lidnode = LID && (m & LBM_MASK_LID);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    C v = St::cvt(ld.raw[i]);
+    if (LID && i > 0) {  // the rest direction never crosses a face
+      const bool wall = lidnode && ((m >> i) & 1u);
+      v += wall ? k.lid[i] : (C)0;
+    }
+    gv[i] = v;
+  }
+
+  // moments; with shifted storage sum w_i = 1 and sum w_i e_i = 0
+  C rho = kSh ? (C)1 : (C)0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    rho += gv[i];
+    if (L::E[i][0]) jx += L::E[i][0] > 0 ? gv[i] : -gv[i];
+    if (L::E[i][1]) jy += L::E[i][1] > 0 ? gv[i] : -gv[i];
+    if (L::E[i][2]) jz += L::E[i][2] > 0 ? gv[i] : -gv[i];
+  }
+  const C ir = recip(rho);
+  const C ux = (jx + k.half_force[0]) * ir;
+  const C uy = (jy + k.half_force[1]) * ir;
+  const C uz = (jz + k.half_force[2]) * ir;
+  const C usq = ux * ux + uy * uy + uz * uz;
+  const C m3uF = (C)-3 * (ux * k.force[0] + uy * k.force[1] + uz * k.force[2]);
+  const C base = (C)1 - (C)1.5 * usq;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    // e.u from the nonzero components only (-0 + v == v exactly)
+    C eu = (C)-0.0;
+    if (L::E[i][0]) eu = L::E[i][0] > 0 ? eu + ux : eu - ux;
+    if (L::E[i][1]) eu = L::E[i][1] > 0 ? eu + uy : eu - uy;
+    if (L::E[i][2]) eu = L::E[i][2] > 0 ? eu + uz : eu - uz;
+    const C w = (C)L::W[i];
+    // t = 1 - 1.5 u.u + 3 e.u + 4.5 (e.u)^2, feq = w rho t
+    const C t = fmadd(fmadd((C)4.5, eu, (C)3), eu, base);
+    // feq - f in the stored form: w rho t - w - g (shifted), w rho t - f
+    const C d = (kSh ? fmadd(w * rho, t, -w) : (w * rho) * t) - gv[i];
+    // Guo source (1 - omega/2) w (3 (e.F - u.F) + 9 (e.u)(e.F))
+    const C src = fmadd(k.gf9[i], eu, fmadd(k.gw[i], m3uF, k.gf3[i]));
+    C out = fmadd(k.omega, d, gv[i]) + src;
+    out = ld.solid ? (kSh ? (C)0 : w) : out;
+    if (NT)
+      St::store_nt(elem(B, ld.own + (uint32_t)i * kTile), out);
+    else
+      St::store(elem(B, ld.own + (uint32_t)i * kTile), out);
+  }
 }
 
 template <int Q, int S, bool LID, bool NT = false>
@@ -40,86 +195,27 @@ __device__ __forceinline__ void std_node_step(
     const Grid& g, const typename Store<S>::T* __restrict__ A,
     typename Store<S>::T* __restrict__ B, const uint32_t* __restrict__ mask,
     const uint32_t* __restrict__ cls, const StdParams& p, uint32_t n) {
-  using L = Lat<Q>;
-  using St = Store<S>;
-  using C = typename St::C;
-  const uint32_t c = node_class(cls, n);
-  uint32_t x, y, z;
-  decode_node(g, n, x, y, z);
-  const uint32_t plane = g.ny * g.nz;
-  // source coordinate c - e for e = -1, 0, +1 (periodic wrap; a wall link
-  // is flagged in the mask and its wrapped value is replaced)
-  const uint32_t sx[3] = {(x + 1 == g.nx ? 0u : x + 1) * plane, x * plane,
-                          (x == 0 ? g.nx : x) * plane - plane};
-  const uint32_t sy[3] = {(y + 1 == g.ny ? 0u : y + 1) * g.nz, y * g.nz,
-                          (y == 0 ? g.ny : y) * g.nz - g.nz};
-  const uint32_t sz[3] = {z + 1 == g.nz ? 0u : z + 1, z,
-                          (z == 0 ? g.nz : z) - 1};
+  NodeLoad<Q, S> ld;
+  std_gather<Q, S>(g, A, mask, cls, n, ld);
+  std_collide<Q, S, LID, NT>(p, B, ld);
+}
 
-  const bool solid = c == kClassSolid;
-  const uint32_t m = c == kClassBoundary ? mask[n] : 0u;
-  C f[Q];
-  if (!solid) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      const uint32_t src = sx[L::E[i][0] + 1] + sy[L::E[i][1] + 1] + sz[L::E[i][2] + 1];
-      const bool bb = (m >> i) & 1u;
-      C v = St::load(A + (bb ? pidx<Q>(n, L::INV[i]) : pidx<Q>(src, i)));
-      if (LID) {
-        const int ex = L::E[i][0], ey = L::E[i][1], ez = L::E[i][2];
-        const int el = p.lid_axis == 0 ? ex : (p.lid_axis == 1 ? ey : ez);
-        if (bb && (m & LBM_MASK_LID) && el == -p.lid_sign) {
-          const C eu = (C)ex * (C)p.lid_u[0] + (C)ey * (C)p.lid_u[1] +
-                       (C)ez * (C)p.lid_u[2];
-          // shifted storage: the wall term is added to f, not to f - w
-          v += (C)6 * (C)L::W[i] * (C)p.wall_rho * eu;
-        }
-      }
-      f[i] = v;
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = St::kShifted ? (C)0 : (C)L::W[i];
-  }
-  if (St::kShifted) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] += (C)L::W[i];
-  }
-
-  C rho = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    rho += f[i];
-    if (L::E[i][0]) jx += L::E[i][0] > 0 ? f[i] : -f[i];
-    if (L::E[i][1]) jy += L::E[i][1] > 0 ? f[i] : -f[i];
-    if (L::E[i][2]) jz += L::E[i][2] > 0 ? f[i] : -f[i];
-  }
-  const C omega = (C)1 / (C)p.tau;
-  const C guo = (C)1 - (C)0.5 * omega;
-  const C Fx = (C)p.force[0], Fy = (C)p.force[1], Fz = (C)p.force[2];
-  const C ir = (C)1 / rho;
-  const C ux = (jx + (C)0.5 * Fx) * ir;
-  const C uy = (jy + (C)0.5 * Fy) * ir;
-  const C uz = (jz + (C)0.5 * Fz) * ir;
-  const C usq = ux * ux + uy * uy + uz * uz;
-  const C uF = ux * Fx + uy * Fy + uz * Fz;
-  const C base = (C)1 - (C)1.5 * usq;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const C ex = (C)L::E[i][0], ey = (C)L::E[i][1], ez = (C)L::E[i][2];
-    const C eu = ex * ux + ey * uy + ez * uz;
-    const C eF = ex * Fx + ey * Fy + ez * Fz;
-    const C w = (C)L::W[i];
-    const C feq = w * rho * (base + (C)3 * eu + (C)4.5 * eu * eu);
-    const C src = guo * w * ((C)3 * (eF - uF) + (C)9 * eu * eF);
-    C out = f[i] + omega * (feq - f[i]) + src;
-    out = solid ? w : out;
-    if (St::kShifted) out -= w;
-    if (NT)
-      St::store_nt(B + pidx<Q>(n, i), out);
-    else
-      St::store(B + pidx<Q>(n, i), out);
-  }
+// Two nodes per thread: both nodes' loads are issued before either is
+// collided, doubling the bytes in flight per wavefront (16-bit storage
+// moves too few bytes per load instruction to cover HBM latency with one
+// node per thread).  Nodes past the lattice end (n >= N) load node N - 1
+// and store nothing.
+template <int Q, int S, bool LID, bool NT>
+__device__ __forceinline__ void std_node_step2(
+    const Grid& g, const typename Store<S>::T* __restrict__ A,
+    typename Store<S>::T* __restrict__ B, const uint32_t* __restrict__ mask,
+    const uint32_t* __restrict__ cls, const StdParams& p, uint32_t n0, uint32_t n1) {
+  NodeLoad<Q, S> a, b;
+  const uint32_t last = g.n - 1;
+  std_gather<Q, S>(g, A, mask, cls, n0 < g.n ? n0 : last, a);
+  std_gather<Q, S>(g, A, mask, cls, n1 < g.n ? n1 : last, b);
+  if (n0 < g.n) std_collide<Q, S, LID, NT>(p, B, a);
+  if (n1 < g.n) std_collide<Q, S, LID, NT>(p, B, b);
 }
 
 }  // namespace lbm

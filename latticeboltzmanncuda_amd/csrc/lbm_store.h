@@ -16,6 +16,7 @@
 #include <cstdint>
 
 #include "../../include/lbm.h"
+#include "lbm_global.h"
 #include "lbm_internal.h"
 
 namespace lbm {
@@ -28,9 +29,11 @@ struct Store<LBM_STORE_F32> {
   using T = float;  // stored
   using C = float;  // computed
   static constexpr bool kShifted = false;
-  __device__ static C load(const T* p) { return *p; }
-  __device__ static void store(T* p, C v) { *p = v; }
-  __device__ static void store_nt(T* p, C v) { __builtin_nontemporal_store(v, p); }
+  __device__ static C cvt(T v) { return v; }
+  __device__ static T rest(int, double w) { return (T)w; }
+  __device__ static C load(const T* p) { return gload(p); }
+  __device__ static void store(T* p, C v) { gstore(p, v); }
+  __device__ static void store_nt(T* p, C v) { gstore_nt(p, v); }
 };
 
 template <>
@@ -38,9 +41,11 @@ struct Store<LBM_STORE_F64> {
   using T = double;
   using C = double;
   static constexpr bool kShifted = false;
-  __device__ static C load(const T* p) { return *p; }
-  __device__ static void store(T* p, C v) { *p = v; }
-  __device__ static void store_nt(T* p, C v) { __builtin_nontemporal_store(v, p); }
+  __device__ static C cvt(T v) { return v; }
+  __device__ static T rest(int, double w) { return w; }
+  __device__ static C load(const T* p) { return gload(p); }
+  __device__ static void store(T* p, C v) { gstore(p, v); }
+  __device__ static void store_nt(T* p, C v) { gstore_nt(p, v); }
 };
 
 template <>
@@ -48,11 +53,11 @@ struct Store<LBM_STORE_F16> {
   using T = _Float16;
   using C = float;
   static constexpr bool kShifted = true;
-  __device__ static C load(const T* p) { return (float)*p; }
-  __device__ static void store(T* p, C v) { *p = (_Float16)v; }
-  __device__ static void store_nt(T* p, C v) {
-    __builtin_nontemporal_store((_Float16)v, p);
-  }
+  __device__ static C cvt(T v) { return (float)v; }
+  __device__ static T rest(int, double) { return (T)0; }
+  __device__ static C load(const T* p) { return (float)gload(p); }
+  __device__ static void store(T* p, C v) { gstore(p, (_Float16)v); }
+  __device__ static void store_nt(T* p, C v) { gstore_nt(p, (_Float16)v); }
 };
 
 template <>
@@ -60,18 +65,16 @@ struct Store<LBM_STORE_BF16> {
   using T = uint16_t;
   using C = float;
   static constexpr bool kShifted = true;
-  __device__ static C load(const T* p) {
-    return __uint_as_float(((uint32_t)*p) << 16);
-  }
+  __device__ static C cvt(T v) { return __uint_as_float(((uint32_t)v) << 16); }
+  __device__ static T rest(int, double) { return 0; }
+  __device__ static C load(const T* p) { return cvt(gload(p)); }
   __device__ static T round(C v) {
     uint32_t u = __float_as_uint(v);
     u += 0x7fffu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
   }
-  __device__ static void store(T* p, C v) { *p = round(v); }
-  __device__ static void store_nt(T* p, C v) {
-    __builtin_nontemporal_store(round(v), p);
-  }
+  __device__ static void store(T* p, C v) { gstore(p, round(v)); }
+  __device__ static void store_nt(T* p, C v) { gstore_nt(p, round(v)); }
 };
 
 inline size_t storage_bytes(int storage) {
@@ -86,6 +89,25 @@ inline size_t storage_bytes(int storage) {
     default:
       return 0;
   }
+}
+
+// Element idx of a population array.  Arrays hold fewer than 2^31 elements
+// (lbm_create checks), so for 16-bit storage the byte offset fits 32 bits
+// and the address is base (scalar) + 32-bit offset: one vector instruction
+// instead of a 64-bit add.
+template <class T>
+__device__ __forceinline__ const T* elem(const T* base, uint32_t idx) {
+  if constexpr (sizeof(T) == 2)
+    return (const T*)((const char*)base + (uint32_t)(idx * 2u));
+  else
+    return base + idx;
+}
+template <class T>
+__device__ __forceinline__ T* elem(T* base, uint32_t idx) {
+  if constexpr (sizeof(T) == 2)
+    return (T*)((char*)base + (uint32_t)(idx * 2u));
+  else
+    return base + idx;
 }
 
 // Non-temporal stores pay off once both population copies (2 x bytes of

@@ -52,8 +52,16 @@ def parse():
                     help="comma list of the extra sections to run")
     ap.add_argument("--c3-steps", type=int, default=2000)
     ap.add_argument("--c4-steps", type=int, default=500)
+    ap.add_argument("--c4-launch", default="",
+                    help="block,grid of the batch kernel instead of the "
+                         "autotuner's choice")
     ap.add_argument("--c5-requests", type=int, default=64)
-    ap.add_argument("--c5-lambda", type=float, default=100.0)
+    ap.add_argument("--c5-lambda", default="100",
+                    help="comma list of arrival rates (requests/s)")
+    ap.add_argument("--c5-engines", default="batched,per_lattice",
+                    help="batched, per_lattice, batched_capped (admission "
+                         "also capped at --c5-node-cap resident nodes)")
+    ap.add_argument("--c5-node-cap", type=int, default=32 << 20)
     ap.add_argument("--c5-max-steps", type=int, default=20000)
     return ap.parse_args()
 
@@ -74,7 +82,7 @@ def precision_section(steps):
             "runs": runs}
 
 
-def batched_section(steps):
+def batched_section(steps, launch=None):
     """512 D2Q9 cavity/channel lattices (seed 4): plain per-lattice
     launches vs one persistent launch per step in hipGraph chunks of 100,
     FP32 and FP16 storage; FP16 velocity error against FP64 storage."""
@@ -92,7 +100,7 @@ def batched_section(steps):
         log(f"batched: {name} plain")
         plain = workloads.c4_plain_mlups(specs, steps)
         log(f"batched: {name} persistent")
-        bat = workloads.c4_batched_mlups(specs, steps, 100, True, st)
+        bat = workloads.c4_batched_mlups(specs, steps, 100, True, st, launch)
         out[name] = {
             "plain_mlups": round(plain["mlups"], 1),
             "batched_mlups": round(bat["mlups"], 1),
@@ -109,63 +117,42 @@ def batched_section(steps):
     return out
 
 
-def serving_section(nreq, lam, max_steps):
+def serving_section(nreq, lams, max_steps, engines, node_cap):
     """Continuous-batching solve server, FP16 storage, Poisson arrivals:
-    batched engine vs per-lattice launches on the same requests."""
+    batched engine vs per-lattice launches on the same requests, for each
+    arrival rate."""
     from latticeboltzmanncuda_amd import capi, workloads
-    reqs = workloads.c5_requests(lam, nreq, max_steps=max_steps)
-    out = {"workload": f"{nreq} requests (seed 5), Poisson lambda {lam}/s, "
-                       "50% D2Q9 64-1024^2 squares, 50% D3Q19 32-128^3 "
-                       "spheres, 5-25% solid, tau 0.55-1.0, body-force "
-                       f"channel, residual 1e-6 or {max_steps} steps, "
-                       "200 GB budget, FP16 storage, chunk 100"}
-    for name, eng in (("batched", capi.ENGINE_BATCHED),
-                      ("per_lattice", capi.ENGINE_PER_LATTICE)):
-        log(f"serving: {name}")
-        st, res = capi.serve(reqs, storage=capi.STORE_F16, chunk=100,
-                             engine=eng)
-        conv = sum(1 for r in res if r["status"] == 0)
-        out[name] = {
-            "mlups": round(st["mlups"], 1),
-            "p50_latency_s": round(st["p50_latency_s"], 4),
-            "p99_latency_s": round(st["p99_latency_s"], 4),
-            "mean_latency_s": round(st["mean_latency_s"], 4),
-            "wall_s": round(st["wall_s"], 3),
-            "completed": st["completed"], "converged": conv,
-            "rejected": st["rejected"], "max_resident": st["max_resident"],
-            "peak_gb": round(st["peak_bytes"] / 1e9, 3),
-        }
+    out = {"workload": f"{nreq} requests per rate (seed 5), Poisson "
+                       "arrivals, 50% D2Q9 64-1024^2 squares, 50% D3Q19 "
+                       "32-128^3 spheres, 5-25% solid, tau 0.55-1.0, "
+                       "body-force channel, residual 1e-6 or "
+                       f"{max_steps} steps, 200 GB budget, FP16 storage, "
+                       "chunk 100", "rates": {}}
+    eng = {"batched": (capi.ENGINE_BATCHED, 0),
+           "batched_capped": (capi.ENGINE_BATCHED, node_cap),
+           "per_lattice": (capi.ENGINE_PER_LATTICE, 0)}
+    for lam in lams:
+        reqs = workloads.c5_requests(lam, nreq, max_steps=max_steps)
+        row = {}
+        for name in engines:
+            log(f"serving: lambda {lam} {name}")
+            st, res = capi.serve(reqs, storage=capi.STORE_F16, chunk=100,
+                                 engine=eng[name][0], progress_s=30.0,
+                                 max_resident_nodes=eng[name][1])
+            conv = sum(1 for r in res if r["status"] == 0)
+            row[name] = {
+                "mlups": round(st["mlups"], 1),
+                "p50_latency_s": round(st["p50_latency_s"], 4),
+                "p99_latency_s": round(st["p99_latency_s"], 4),
+                "mean_latency_s": round(st["mean_latency_s"], 4),
+                "wall_s": round(st["wall_s"], 3),
+                "completed": st["completed"], "converged": conv,
+                "rejected": st["rejected"],
+                "max_resident": st["max_resident"],
+                "peak_gb": round(st["peak_bytes"] / 1e9, 3),
+            }
+        out["rates"][str(lam)] = row
     return out
-
-
-def cpu_baseline(solid: np.ndarray, seconds: float) -> dict:
-    """The oracle's FP32 standard-mode step on the host cores, same
-    workload, timed over a bounded number of steps."""
-    from oracle import oracle as O
-
-    q = 19
-    mask = O.std_build_mask(q, DIMS, solid, (1, 0, 0))
-    p = O.std_params(tau=TAU, force=FORCE, periodic=(1, 0, 0))
-    f = O.std_init_eq(q, DIMS).astype(np.float32)
-    n = int(np.prod(DIMS))
-    threads = O.get_threads()
-    f = O.std_run_f32(q, DIMS, f, mask, p, 1)  # warm-up step
-    steps, t0 = 0, time.perf_counter()
-    chunk = 2
-    while time.perf_counter() - t0 < seconds:
-        f = O.std_run_f32(q, DIMS, f, mask, p, chunk)
-        steps += chunk
-    dt = time.perf_counter() - t0
-    return {
-        "value": round(n * steps / dt / 1e6, 2),
-        "unit": "MLUPS",
-        "cores": threads,
-        "kind": "port",
-        "sample": (f"oracle FP32 standard-mode step (C, OpenMP {threads} "
-                   f"threads) on the same 256x128x128 sphere channel, "
-                   f"{steps} steps in {dt:.1f} s; includes a per-call "
-                   f"population copy-in/out every {chunk} steps"),
-    }
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles",
@@ -264,11 +251,14 @@ def main():
         out["precision"] = precision_section(args.c3_steps)
     if "batched" in sections:
         log("batched")
-        out["batched"] = batched_section(args.c4_steps)
+        launch = (tuple(int(v) for v in args.c4_launch.split(","))
+                  if args.c4_launch else None)
+        out["batched"] = batched_section(args.c4_steps, launch)
     if "serving" in sections:
         log("serving")
-        out["serving"] = serving_section(args.c5_requests, args.c5_lambda,
-                                         args.c5_max_steps)
+        out["serving"] = serving_section(
+            args.c5_requests, [float(v) for v in args.c5_lambda.split(",")],
+            args.c5_max_steps, args.c5_engines.split(","), args.c5_node_cap)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(solid, args.cpu_seconds)
     print(json.dumps(out))

@@ -309,6 +309,10 @@ typedef struct lbm_serve_params {
   int32_t max_resident; /* 0 = no limit */
   int32_t autotune;     /* pick the launch shape on a scratch batch first */
   int32_t engine;       /* LBM_ENGINE_BATCHED or LBM_ENGINE_PER_LATTICE */
+  double progress_s;    /* > 0: a status line on stderr this often */
+  int64_t max_resident_nodes; /* 0 = no limit; otherwise admit only while
+                                 the resident lattices hold fewer nodes
+                                 (a lattice alone is always admitted) */
 } lbm_serve_params;
 
 /* how a server advances its resident lattices by one chunk */

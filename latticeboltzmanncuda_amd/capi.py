@@ -97,6 +97,8 @@ class ServeParams(C.Structure):
         ("max_resident", C.c_int32),
         ("autotune", C.c_int32),
         ("engine", C.c_int32),
+        ("progress_s", C.c_double),
+        ("max_resident_nodes", C.c_int64),
     ]
 
 
@@ -254,11 +256,20 @@ def make_config(nx, ny, nz, lattice=D3Q19, storage=STORE_F32,
     return c
 
 
+def copy_config(cfg: Config) -> Config:
+    c = Config()
+    C.pointer(c)[0] = cfg
+    return c
+
+
 class Context:
-    """One lattice resident on the GPU (an lbm_ctx handle)."""
+    """One lattice resident on the GPU (an lbm_ctx handle).  Keeps its own
+    copy of the configuration (set_tau / set_body_force update the copy, not
+    the caller's object)."""
 
     def __init__(self, cfg: Config, stream: int | None = None):
         self.lib = load()
+        cfg = copy_config(cfg)
         self.cfg = cfg
         h = CTX()
         check(self.lib.lbm_create(C.byref(cfg), stream, C.byref(h)),
@@ -372,6 +383,16 @@ class Context:
     def flow(self):
         check(self.lib.lbm_flow(self.h), "lbm_flow")
 
+    def set_tau(self, tau: float):
+        check(self.lib.lbm_set_tau(self.h, float(tau)), "lbm_set_tau")
+        self.cfg.tau = float(tau)
+
+    def set_body_force(self, force):
+        a = np.ascontiguousarray(force, np.float32)
+        check(self.lib.lbm_set_body_force(self.h, _ptr(a)),
+              "lbm_set_body_force")
+        self.cfg.force[:] = [float(v) for v in a]
+
     def set_force(self, f):
         a = np.ascontiguousarray(f, np.float32)
         check(self.lib.lbm_set_force(self.h, _ptr(a)), "lbm_set_force")
@@ -432,7 +453,7 @@ class Member(Context):
 
     def __init__(self, batch, slot, handle, cfg):
         self.lib = load()
-        self.cfg = cfg
+        self.cfg = cfg = copy_config(cfg)
         self.h = handle
         self.q = cfg.lattice
         self.dims = (cfg.nx, cfg.ny, cfg.nz)
@@ -554,7 +575,7 @@ def step_each(contexts, n: int) -> None:
 
 def serve(requests, storage=STORE_F16, budget_bytes=200 * 10 ** 9,
           chunk=100, max_resident=0, autotune=True, keep_results=False,
-          engine=ENGINE_BATCHED):
+          engine=ENGINE_BATCHED, progress_s=0.0, max_resident_nodes=0):
     """Runs lbm_serve_run on a list of dicts with keys cfg (Config), solid
     (array or None), arrival_s, max_steps, tol.  engine: ENGINE_BATCHED
     (persistent kernel, hipGraph chunks) or ENGINE_PER_LATTICE (one plain
@@ -584,6 +605,8 @@ def serve(requests, storage=STORE_F16, budget_bytes=200 * 10 ** 9,
     p.storage, p.budget_bytes, p.chunk = storage, budget_bytes, chunk
     p.max_resident, p.autotune = max_resident, int(autotune)
     p.engine = engine
+    p.progress_s = progress_s
+    p.max_resident_nodes = max_resident_nodes
     st = ServeStats()
     check(load().lbm_serve_run(C.byref(p), arr, n, C.byref(st)),
           "lbm_serve_run")

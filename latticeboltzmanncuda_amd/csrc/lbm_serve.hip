@@ -6,8 +6,11 @@
 //   1. moves arrived requests to a FIFO;
 //   2. admits from the FIFO head while the batch's HBM budget (both
 //      population buffers, geometry, residual snapshot) and the resident
-//      limit allow -- requests that arrive while others run join at the
-//      next chunk boundary as memory frees up;
+//      limits (lattices, nodes) allow -- requests that arrive while others
+//      run join at the next chunk boundary as memory frees up.  A node
+//      limit a little above what fills the GPU keeps the throughput and
+//      lets early requests finish first instead of sharing the GPU with
+//      every later one;
 //   3. advances every resident lattice by one chunk: with the batch's
 //      persistent kernel (replayed hipGraph), or -- the comparison
 //      engine -- with one plain launch per lattice per step;
@@ -22,6 +25,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <deque>
 #include <thread>
 #include <vector>
@@ -109,6 +113,7 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
   std::deque<int32_t> fifo;
   std::vector<int32_t> slot_req;  // slot -> request (-1 free)
   int32_t resident = 0, next = 0, finished = 0;
+  int64_t resident_nodes = 0;
   std::vector<double> residuals;
   std::vector<float> scratch;
   std::vector<lbm_ctx*> members;
@@ -135,12 +140,20 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
     if ((e = lbm_batch_remove(b, slot))) return e;
     slot_req[slot] = -1;
     --resident;
+    resident_nodes -= (int64_t)reqs[r].cfg.nx * reqs[r].cfg.ny * reqs[r].cfg.nz;
     ++finished;
     return LBM_OK;
   };
 
+  double next_report = p->progress_s > 0.0 ? p->progress_s : INFINITY;
   while (finished < nreq) {
     const double now = seconds_since(t0);
+    if (now >= next_report) {
+      std::fprintf(stderr, "lbm_serve: t=%.1fs finished %d/%d resident %d queued %zu\n", now,
+                   finished, nreq, resident, fifo.size() + (size_t)(nreq - next));
+      std::fflush(stderr);
+      next_report = now + p->progress_s;
+    }
     while (next < nreq && reqs[order[next]].arrival_s <= now) fifo.push_back(order[next++]);
     // admission, FIFO
     while (!fifo.empty()) {
@@ -155,6 +168,10 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
         continue;
       }
       if (p->max_resident > 0 && resident >= p->max_resident) break;
+      const int64_t nodes = (int64_t)reqs[r].cfg.nx * reqs[r].cfg.ny * reqs[r].cfg.nz;
+      if (p->max_resident_nodes > 0 && resident > 0 &&
+          resident_nodes + nodes > p->max_resident_nodes)
+        break;
       int32_t slot = -1;
       const int e = lbm_batch_add(b, &reqs[r].cfg, reqs[r].solid, 1, &slot);
       if (e == LBM_ERR_BUDGET && resident > 0) break;  // wait for memory to free up
@@ -178,6 +195,7 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
       if (t_first < 0.0) t_first = reqs[r].t_admit_s;
       fifo.pop_front();
       ++resident;
+      resident_nodes += nodes;
       stats->max_resident = std::max(stats->max_resident, resident);
       int64_t bytes = 0;
       lbm_batch_info(b, nullptr, nullptr, &bytes);

@@ -82,7 +82,6 @@ __device__ __forceinline__ void std_gather(const Grid& g,
                                            const uint32_t* __restrict__ cls, uint32_t n,
                                            NodeLoad<Q, S>& ld) {
   using L = Lat<Q>;
-  using St = Store<S>;
   const uint32_t c = node_class(cls, n);
   uint32_t x, y, z;
   if (Q == 9) {  // D2Q9 lattices have nz == 1
@@ -106,9 +105,10 @@ __device__ __forceinline__ void std_gather(const Grid& g,
   ld.solid = c == kClassSolid;
   ld.m = c == kClassBoundary ? gload(mask + n) : 0u;
   ld.own = pidx<Q>(n, 0);
-  // solid lanes load nothing and keep the rest state (f = w_i, g = 0)
+  // solid lanes load nothing and keep the rest state (f = w_i, g = 0):
+  // zero for shifted storage; unshifted storage selects w_i when colliding
 #pragma unroll
-  for (int i = 0; i < Q; ++i) ld.raw[i] = St::rest(i, L::W[i]);
+  for (int i = 0; i < Q; ++i) ld.raw[i] = 0;
   if (!ld.solid) {
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -144,6 +144,7 @@ __device__ __forceinline__ void std_collide(const StdParams& p,
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     C v = St::cvt(ld.raw[i]);
+    if (!kSh) v = ld.solid ? (C)L::W[i] : v;
     if (LID && i > 0) {  // the rest direction never crosses a face
       const bool wall = lidnode && ((m >> i) & 1u);
       v += wall ? k.lid[i] : (C)0;

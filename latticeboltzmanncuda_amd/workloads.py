@@ -230,17 +230,22 @@ def c4_plain_mlups(specs, steps: int) -> dict:
 
 
 def c4_batched_mlups(specs, steps: int, chunk: int = 100,
-                     autotune: bool = True, storage: int = capi.STORE_F32
-                     ) -> dict:
-    """All lattices in one batch: one persistent launch per step, replayed
-    as hipGraph chunks of `chunk` steps, launch shape from the autotuner;
-    wall-clock MLUPS (hipEvents agree within launch jitter)."""
+                     autotune: bool = True, storage: int = capi.STORE_F32,
+                     launch=None) -> dict:
+    """All lattices in one batch: one persistent launch per kernel class and
+    step, replayed as hipGraph chunks of `chunk` steps, launch shape from
+    the autotuner (or `launch` = (block, grid)); wall-clock MLUPS (hipEvents
+    agree within launch jitter)."""
     import time
     with capi.Batch(storage) as b:
         for cfg, solid, _ in specs:
             b.add(cfg, solid)
         _, nodes, nbytes = b.info()
-        block, grid = b.autotune(steps=10) if autotune else b.launch()
+        if launch is not None:
+            b.set_launch(*launch)
+            block, grid = launch
+        else:
+            block, grid = b.autotune(steps=10) if autotune else b.launch()
         b.step(chunk, chunk)  # capture + warm-up
         b.synchronize()
         t0 = time.perf_counter()

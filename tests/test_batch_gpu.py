@@ -142,3 +142,27 @@ def test_batch_residuals_match_host():
             ref = np.sqrt((d ** 2).sum() / (u.astype(np.float64)[:, fl] ** 2).sum())
             assert abs(r0[m.slot] - 1.0) < 1e-9
             assert abs(r1[m.slot] - ref) <= 1e-6 * ref + 1e-12
+
+
+def test_batch_member_parameter_change_between_chunks():
+    say("case: tau / force of a member changed between chunks")
+    cfg, solid = mixed_specs()[1]  # D2Q9 channel with squares
+    with capi.Batch(capi.STORE_F32) as b:
+        m = b.add(cfg, solid)
+        b.step(20, 10)
+        m.set_tau(0.65)
+        m.set_body_force((2e-5, 0.0, 0.0))
+        b.step(20, 10)
+        b.synchronize()
+        got = m.populations()
+    with capi.Context(capi.make_config(
+            cfg.nx, cfg.ny, cfg.nz, lattice=cfg.lattice, tau=cfg.tau,
+            periodic=tuple(cfg.periodic), force=tuple(cfg.force))) as c:
+        c.upload_solid(solid)
+        c.init_equilibrium()
+        c.step(20)
+        c.set_tau(0.65)
+        c.set_body_force((2e-5, 0.0, 0.0))
+        c.step(20)
+        ref = c.populations()
+    assert np.array_equal(got, ref)

@@ -92,6 +92,25 @@ def test_serve_budget_queues_and_rejects():
         assert np.array_equal(a["u"], b["u"])
 
 
+def test_serve_node_cap_queues_without_changing_results():
+    say("case: serve with a resident-node cap")
+    reqs = make_requests(n=8, seed=12)
+    for r in reqs:
+        r["arrival_s"] = 0.0  # all queued at once
+    cap = max(r["cfg"].nx * r["cfg"].ny * r["cfg"].nz for r in reqs) + 1
+    st_cap, res_cap = capi.serve(reqs, storage=capi.STORE_F32, chunk=100,
+                                 keep_results=True, max_resident_nodes=cap)
+    st, res = capi.serve(reqs, storage=capi.STORE_F32, chunk=100,
+                         keep_results=True)
+    say(f"  capped max_resident {st_cap['max_resident']}, "
+        f"uncapped {st['max_resident']}")
+    assert st_cap["completed"] == len(reqs)
+    assert st_cap["max_resident"] < st["max_resident"]
+    for a, b in zip(res_cap, res):
+        assert a["steps"] == b["steps"] and a["status"] == b["status"]
+        assert np.array_equal(a["u"], b["u"])
+
+
 def test_serve_fp16_accuracy_vs_fp64():
     say("case: serve FP16 storage vs FP64 storage")
     r16 = make_requests(storage=capi.STORE_F16, n=6, seed=9)

@@ -155,6 +155,36 @@ def serving_section(nreq, lams, max_steps, engines, node_cap):
     return out
 
 
+def cpu_baseline(solid: np.ndarray, seconds: float) -> dict:
+    """The oracle's FP32 standard-mode step on the host cores, same
+    workload, timed over a bounded number of steps."""
+    from oracle import oracle as O
+
+    q = 19
+    mask = O.std_build_mask(q, DIMS, solid, (1, 0, 0))
+    p = O.std_params(tau=TAU, force=FORCE, periodic=(1, 0, 0))
+    f = O.std_init_eq(q, DIMS).astype(np.float32)
+    n = int(np.prod(DIMS))
+    threads = O.get_threads()
+    f = O.std_run_f32(q, DIMS, f, mask, p, 1)  # warm-up step
+    steps, t0 = 0, time.perf_counter()
+    chunk = 8
+    while time.perf_counter() - t0 < seconds:
+        f = O.std_run_f32(q, DIMS, f, mask, p, chunk)
+        steps += chunk
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(n * steps / dt / 1e6, 2),
+        "unit": "MLUPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle FP32 standard-mode step (C, OpenMP {threads} "
+                   f"threads) on the same 256x128x128 sphere channel, "
+                   f"{steps} steps in {dt:.1f} s; includes a per-call "
+                   f"population copy-in/out every {chunk} steps"),
+    }
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles",
                             "k_std_step_d3q19_f32_channel_traffic.json")
 

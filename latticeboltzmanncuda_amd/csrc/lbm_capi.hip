@@ -220,6 +220,102 @@ int validate_config(const lbm_config* cfg) {
   return LBM_OK;
 }
 
+// Tables of the fused reference step.  Every slot s (1..80) of a
+// reflecting node x with direction d (1..26) whose target t = x + e_d lies
+// in the box becomes an entry of receiver t for direction d; entries of one
+// (t, d) keep ascending s, the order in which the staged reflection pass
+// adds them.  Targets must not be flagged: the staged pass would otherwise
+// race between adding to a target and zeroing it.
+int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
+  const Grid& g = c->grid;
+  const int64_t N = g.n;
+  const size_t words = (size_t)((N + 15) / 16);
+  dev_free(c, (void**)&c->ref_cls, sizeof(uint32_t) * words);
+  dev_free(c, (void**)&c->ref_inmask, sizeof(uint32_t) * (size_t)N);
+  dev_free(c, (void**)&c->ref_rxbase, sizeof(int32_t) * (size_t)N);
+  dev_free(c, (void**)&c->ref_rxent, sizeof(uint32_t) * (size_t)std::max<int64_t>(c->ref_nent, 1));
+  dev_free(c, (void**)&c->ref_rxw, sizeof(float) * (size_t)std::max<int64_t>(c->ref_nent, 1));
+  dev_free(c, (void**)&c->ref_partials, sizeof(float) * 3 * (size_t)std::max(c->ref_partials_cap, 1));
+  c->ref_nent = 0;
+  c->ref_partials_cap = 0;
+  c->ref_geom = c->nspecial > 0;
+  if (!c->ref_geom) return LBM_OK;
+
+  struct Link {
+    uint32_t t;   // receiving node
+    uint8_t d;    // direction of the reflected population
+    uint8_t s;    // slot of the reflecting node's record
+    int32_t r;    // record
+  };
+  std::vector<Link> links;
+  std::vector<uint32_t> cls(words, 0u);
+  for (int64_t n = 0; n < N; ++n) {
+    const uint8_t fl = c->host_flags[n];
+    if (fl & LBM_FLAG_INTERNAL) cls[n >> 4] |= 2u << (2 * (n & 15));
+    const int32_t r = c->nrec > 0 ? rec_of[n] : -1;
+    if (!(fl & LBM_FLAG_REFLECTED) || r < 0) continue;
+    const int64_t x = n / ((int64_t)g.ny * g.nz), rem = n % ((int64_t)g.ny * g.nz);
+    const int64_t y = rem / g.nz, z = rem % g.nz;
+    const uint32_t* w = c->host_rec_dirs.data() + 14 * (size_t)r;
+    for (int s = 1; s < 81; ++s) {
+      const int d = (int)((w[s / 6] >> (5 * (s % 6))) & 0x1Fu);
+      if (d == 0 || d > 26) continue;
+      const int64_t tx = x + Lat<27>::E[d][0], ty = y + Lat<27>::E[d][1], tz = z + Lat<27>::E[d][2];
+      if (tx < 0 || ty < 0 || tz < 0 || tx >= g.nx || ty >= g.ny || tz >= g.nz) continue;
+      const int64_t t = tz + (int64_t)g.nz * (ty + (int64_t)g.ny * tx);
+      if (c->host_flags[t])
+        return fail(LBM_ERR_INVALID, "reflection target " + std::to_string(t) +
+                                         " is a flagged node (targets must be fluid)");
+      links.push_back({(uint32_t)t, (uint8_t)d, (uint8_t)s, r});
+    }
+  }
+  std::stable_sort(links.begin(), links.end(), [](const Link& a, const Link& b) {
+    return a.t != b.t ? a.t < b.t : (a.d != b.d ? a.d < b.d : a.s < b.s);
+  });
+  std::vector<uint32_t> inmask((size_t)N, 0u), ent;
+  std::vector<int32_t> base((size_t)N, -1);
+  std::vector<float> wts;
+  for (size_t i = 0; i < links.size();) {
+    const uint32_t t = links[i].t;
+    base[t] = (int32_t)ent.size();
+    cls[t >> 4] |= 1u << (2 * (t & 15));
+    while (i < links.size() && links[i].t == t) {
+      const uint8_t d = links[i].d;
+      inmask[t] |= 1u << d;
+      size_t j = i;
+      while (j < links.size() && links[j].t == t && links[j].d == d) ++j;
+      ent.push_back((uint32_t)(j - i));
+      wts.push_back(0.0f);
+      for (size_t k = i; k < j; ++k) {
+        const int s = links[k].s;
+        ent.push_back((uint32_t)(s / 3) | ((uint32_t)s << 5));
+        wts.push_back(c->host_rec_w[81 * (size_t)links[k].r + (size_t)s]);
+      }
+      i = j;
+    }
+  }
+  if (ent.size() >= (1ull << 31)) return fail(LBM_ERR_UNSUPPORTED, "too many reflected links");
+  c->ref_nent = (int64_t)ent.size();
+  c->ref_partials_cap = ref_full_blocks(g.n);
+  int rc;
+  if ((rc = dev_alloc(c, (void**)&c->ref_cls, sizeof(uint32_t) * words, "hipMalloc(ref classes)")) ||
+      (rc = dev_alloc(c, (void**)&c->ref_inmask, sizeof(uint32_t) * (size_t)N, "hipMalloc(ref in-masks)")) ||
+      (rc = dev_alloc(c, (void**)&c->ref_rxbase, sizeof(int32_t) * (size_t)N, "hipMalloc(ref receivers)")) ||
+      (rc = dev_alloc(c, (void**)&c->ref_rxent, sizeof(uint32_t) * std::max<size_t>(ent.size(), 1), "hipMalloc(ref entries)")) ||
+      (rc = dev_alloc(c, (void**)&c->ref_rxw, sizeof(float) * std::max<size_t>(wts.size(), 1), "hipMalloc(ref weights)")) ||
+      (rc = dev_alloc(c, (void**)&c->ref_partials, sizeof(float) * 3 * (size_t)c->ref_partials_cap, "hipMalloc(ref force partials)")))
+    return rc;
+  if ((rc = h2d(c, c->ref_cls, cls.data(), sizeof(uint32_t) * words)) ||
+      (rc = h2d(c, c->ref_inmask, inmask.data(), sizeof(uint32_t) * (size_t)N)) ||
+      (rc = h2d(c, c->ref_rxbase, base.data(), sizeof(int32_t) * (size_t)N)))
+    return rc;
+  if (!ent.empty() &&
+      ((rc = h2d(c, c->ref_rxent, ent.data(), sizeof(uint32_t) * ent.size())) ||
+       (rc = h2d(c, c->ref_rxw, wts.data(), sizeof(float) * wts.size()))))
+    return rc;
+  return LBM_OK;
+}
+
 // Nodes with a nonzero flag byte and their reflection record (or -1).
 int rebuild_special(lbm_ctx* c) {
   if (!c->special_dirty) return LBM_OK;
@@ -264,6 +360,7 @@ int rebuild_special(lbm_ctx* c) {
     if ((rc = h2d(c, c->special, sp.data(), sizeof(int32_t) * sp.size()))) return rc;
     if ((rc = h2d(c, c->special_rec, sr.data(), sizeof(int32_t) * sr.size()))) return rc;
   }
+  if ((rc = rebuild_receivers(c, rec_of))) return rc;
   c->special_dirty = false;
   return LBM_OK;
 }
@@ -302,7 +399,27 @@ int reflect(lbm_ctx* c, int with_force) {
                             with_force ? c->force : nullptr, c->stream);
 }
 
+// The reference's main loop S -> R -> C -> F (LatticeBoltzmannCuda.cpp:
+// 85-93), one fused pass per step.
 int step_reference_full(lbm_ctx* c, int n, int with_force) {
+  int rc = rebuild_special(c);
+  if (rc) return rc;
+  const bool geom = c->ref_geom;
+  for (int s = 0; s < n; ++s) {
+    if ((rc = launch_ref_step_full(c->grid, (const float*)c->f[c->cur], (float*)c->f[c->cur ^ 1],
+                                   c->cfg.tau, c->flow, geom ? c->ref_cls : nullptr,
+                                   c->ref_inmask, c->ref_rxbase, c->ref_rxent, c->ref_rxw,
+                                   geom && with_force ? c->ref_partials : nullptr,
+                                   geom && with_force ? c->force : nullptr, c->stream)))
+      return rc;
+    c->cur ^= 1;
+  }
+  return LBM_OK;
+}
+
+// The same loop as separate stage passes (stream A -> B, reflect in place,
+// collide + flow in place): the stage kernels the fused pass must match.
+int step_reference_staged(lbm_ctx* c, int n, int with_force) {
   int rc = rebuild_special(c);
   if (rc) return rc;
   for (int s = 0; s < n; ++s) {
@@ -423,7 +540,9 @@ int lbm_destroy(lbm_ctx* c) {
                   (void*)c->rec_node, (void*)c->rec_dirs, (void*)c->rec_w,
                   (void*)c->special, (void*)c->special_rec,
                   (void*)c->flow_mask, (void*)c->force,
-                  (void*)c->force_partials, (void*)c->red, (void*)c->red_out})
+                  (void*)c->force_partials, (void*)c->red, (void*)c->red_out,
+                  (void*)c->ref_cls, (void*)c->ref_inmask, (void*)c->ref_rxbase,
+                  (void*)c->ref_rxent, (void*)c->ref_rxw, (void*)c->ref_partials})
     if (p) (void)hipFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -614,6 +733,8 @@ int lbm_upload_reflections(lbm_ctx* c, int32_t count, const int32_t* node,
   dev_free(c, (void**)&c->rec_w, sizeof(float) * 81 * old);
   c->nrec = 0;
   c->host_rec_node.clear();
+  c->host_rec_dirs.clear();
+  c->host_rec_w.clear();
   c->special_dirty = true;
   const size_t m = (size_t)std::max(count, 1);
   int rc;
@@ -629,6 +750,8 @@ int lbm_upload_reflections(lbm_ctx* c, int32_t count, const int32_t* node,
   }
   c->nrec = count;
   c->host_rec_node.assign(node, node + count);
+  c->host_rec_dirs.assign(dirs, dirs + 14 * (size_t)count);
+  c->host_rec_w.assign(weights, weights + 81 * (size_t)count);
   return LBM_OK;
 }
 

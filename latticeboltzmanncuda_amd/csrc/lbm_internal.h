@@ -123,6 +123,15 @@ int launch_ref_reflect(const Grid& g, float* f, int32_t nspecial,
                        const float* rec_w, float* force_partials,
                        int nblocks, float* force, hipStream_t s);
 int ref_reflect_blocks(int32_t nspecial);
+// Fused stream + reflect + collide + flow (A -> B); geometry tables may be
+// null when no node is flagged; partials (3 per block, ref_full_blocks) and
+// force may be null to skip the force.
+int launch_ref_step_full(const Grid& g, const float* A, float* B, float tau,
+                         const FlowSpec& flow, const uint32_t* cls,
+                         const uint32_t* inmask, const int32_t* rxbase,
+                         const uint32_t* rxent, const float* rxw,
+                         float* partials, float* force, hipStream_t s);
+int ref_full_blocks(uint32_t n);
 int launch_ref_flow(const Grid& g, float* f, const FlowSpec& flow,
                     hipStream_t s);
 int launch_ref_moments(const Grid& g, const float* f, float* rho, float* u,
@@ -162,6 +171,8 @@ struct lbm_ctx {
   uint32_t* rec_dirs = nullptr;
   float* rec_w = nullptr;
   std::vector<int32_t> host_rec_node;
+  std::vector<uint32_t> host_rec_dirs;  // 14 packed words per record
+  std::vector<float> host_rec_w;        // 81 weights per record
   // nodes with a nonzero flag byte, and their record (or -1)
   int32_t nspecial = 0;
   int32_t* special = nullptr;
@@ -176,6 +187,19 @@ struct lbm_ctx {
   float* force = nullptr;
   float* force_partials = nullptr;
   int force_partials_cap = 0;
+
+  // fused reference step (stream + reflect + collide + flow in one pass):
+  // the nodes that receive reflected populations gather them from the
+  // reflecting node's streamed values
+  bool ref_geom = false;           // any flagged node
+  uint32_t* ref_cls = nullptr;     // 2-bit: 0 plain, 1 receiver, 2 internal
+  uint32_t* ref_inmask = nullptr;  // per node: bit j = a reflected link arrives in direction j
+  int32_t* ref_rxbase = nullptr;   // per node: first entry of a receiver
+  uint32_t* ref_rxent = nullptr;   // per receiver, per incoming j: count, then (k | s << 5)
+  float* ref_rxw = nullptr;        // slot weights, parallel to ref_rxent
+  int64_t ref_nent = 0;
+  float* ref_partials = nullptr;   // force partials of the fused step, 3 per block
+  int ref_partials_cap = 0;
   double* red = nullptr;
   int red_cap = 0;
   double* red_out = nullptr;

@@ -271,10 +271,20 @@ def test_p1_reference_main_loop_1000_steps(data_dir, mesh_name, tau):
     assert np.abs(g.reshape(27, -1)[:, internal]).max() == 0.0
 
 
-def test_reference_staged_equals_fused_full_step(data_dir):
-    say("case: staged S,R,C,F == lbm_step_reference")
-    dims, spacing = (20, 20, 20), 0.05
-    flags, recs, flow, f = _p1_setup(data_dir, dims, spacing, "cube_angled.obj")
+@pytest.mark.parametrize("dims,spacing,mesh_name",
+                         [((20, 20, 20), 0.05, "cube_angled.obj"),
+                          ((32, 32, 32), 0.05, "compat_cube.obj"),
+                          ((80, 72, 64), 0.016, "cube_angled.obj")])
+def test_reference_staged_equals_fused_full_step(data_dir, dims, spacing,
+                                                 mesh_name):
+    # lbm_step_reference runs stream + reflect + collide + flow as one pass
+    # whose reflection gathers from the reflecting node's streamed values;
+    # the stage calls run the reference's separate passes.  Populations are
+    # bit-identical; the force's FP32 partial sums are grouped differently
+    # (by receiving node instead of reflecting node).
+    say(f"case: staged S,R,C,F == lbm_step_reference {dims} {mesh_name}")
+    flags, recs, flow, f = _p1_setup(data_dir, dims, spacing, mesh_name)
+    assert len(recs[0]) > 0
     outs = []
     for staged in (True, False):
         with ref_ctx(dims, tau=0.7) as c:
@@ -293,7 +303,8 @@ def test_reference_staged_equals_fused_full_step(data_dir):
                     c.step_reference(1, True)
             outs.append((c.populations(), c.force()))
     assert np.array_equal(outs[0][0], outs[1][0])
-    assert np.array_equal(outs[0][1], outs[1][1])
+    say(f"  force staged {outs[0][1]} fused {outs[1][1]}")
+    assert rel_err(outs[1][1], outs[0][1]) <= 1e-6
 
 
 # ------------------------------------------------ standard mode vs oracle

@@ -9,8 +9,9 @@ the populations already resident in HBM.  Prints one JSON line.  Besides the
 headline it carries three sections (skipped with --headline-only):
 precision (Taylor-Green 256^3 in FP16/BF16/FP32 storage vs FP64),
 batched (512 D2Q9 lattices, per-lattice launches vs the persistent batch
-kernel, FP32 and FP16) and serving (Poisson requests through the
-continuous-batching server, batched vs per-lattice engine).
+kernel, FP32 and FP16), serving (Poisson requests through the
+continuous-batching server, batched vs per-lattice engine) and reference
+(the reference's D3Q27 main loop, fused vs staged).
 
     python bench.py [--gpus 1] [--steps 5000] [--warmup 100]
 """
@@ -48,9 +49,11 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="skip the precision, batched and serving sections")
-    ap.add_argument("--sections", default="precision,batched,serving",
+    ap.add_argument("--sections",
+                    default="precision,batched,serving,reference",
                     help="comma list of the extra sections to run")
     ap.add_argument("--c3-steps", type=int, default=2000)
+    ap.add_argument("--ref-steps", type=int, default=1000)
     ap.add_argument("--c4-steps", type=int, default=500)
     ap.add_argument("--c4-launch", default="",
                     help="block,grid of the batch kernel instead of the "
@@ -152,6 +155,39 @@ def serving_section(nreq, lams, max_steps, engines, node_cap):
                 "peak_gb": round(st["peak_bytes"] / 1e9, 3),
             }
         out["rates"][str(lam)] = row
+    return out
+
+
+def reference_mode_section(steps):
+    """The reference's own main loop (LatticeBoltzmannCuda.cpp:11-95) on a
+    larger lattice: D3Q27 FP32 128^3, the angled cube of the reference's
+    tests voxelised at spacing 0.0125, plane x = 0 injection, force on; the
+    command-line driver runs it fused (one pass per step) and staged (the
+    reference's four passes)."""
+    import subprocess
+    cli = os.path.join(ROOT, "latticeboltzmanncuda_amd", "bin",
+                       "LatticeBoltzmannHip")
+    mesh = os.path.join(ROOT, "tests", "golden", "dataFiles",
+                        "cube_angled.obj")
+    out = {"workload": "D3Q27 FP32 128^3 reference main loop, angled cube "
+                       "(spacing 0.0125), plane injection, force on, tau 0.6",
+           "steps": steps}
+    n = 128 ** 3
+    for name, extra in (("fused", []), ("staged", ["--staged"])):
+        log(f"reference mode: {name}")
+        r = subprocess.run([cli, mesh, "8", "8", "8", "16", "16", "16", "1.0",
+                            "0.6", "0.0125", str(steps), "--quiet", "--time",
+                            *extra], capture_output=True, text=True,
+                           timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference-mode driver failed: {r.stderr}")
+        t = json.loads(r.stdout.strip().splitlines()[-1])
+        us = t["seconds"] / steps * 1e6
+        out[name] = {"mlups": round(t["mlups"], 1),
+                     "us_per_step": round(us, 2),
+                     "reflection_records": t["reflection_records"],
+                     "hbm_frac_216B": round(n * 216 / (us * 1e-6) / 8e12, 4)}
+    out["speedup"] = round(out["fused"]["mlups"] / out["staged"]["mlups"], 3)
     return out
 
 
@@ -284,6 +320,9 @@ def main():
         launch = (tuple(int(v) for v in args.c4_launch.split(","))
                   if args.c4_launch else None)
         out["batched"] = batched_section(args.c4_steps, launch)
+    if "reference" in sections:
+        log("reference mode")
+        out["reference_mode"] = reference_mode_section(args.ref_steps)
     if "serving" in sections:
         log("serving")
         out["serving"] = serving_section(

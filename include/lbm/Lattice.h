@@ -109,6 +109,11 @@ class Lattice {
   void simulateReflections();
   void simulateFlow();
   void simulateLattice();
+  // One step of the reference main loop's simulateStreaming,
+  // simulateReflections, simulateCollision, simulateFlow
+  // (LatticeBoltzmannCuda.cpp:85-93) as one fused pass on the GPU; the
+  // populations are bit-identical to the four calls.
+  void simulateStep();
 
   void insertModel(std::string filename);
   std::array<std::pair<float, int>, 27> distributeVector(Point_3 vector);

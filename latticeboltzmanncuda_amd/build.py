@@ -6,7 +6,8 @@ produces, in-tree under latticeboltzmanncuda_amd/lib/:
   liblbm_hip.so   the C-ABI of include/lbm.h (HIP kernels for gfx950)
   liblbm.so       the host-only C++ facade of include/lbm/*.h (Lattice,
                   Model, Geometry, FlowData ...) over that C-ABI (g++)
-and under latticeboltzmanncuda_amd/bin/ the C++ test suites of tests/cpp/.
+and under latticeboltzmanncuda_amd/bin/ the C++ test suites of tests/cpp/
+and the command-line driver LatticeBoltzmannHip (cli/).
 
 Every .hip file is compiled to an object with hipcc (parallel), then
 linked into one shared library.  Compilation always runs (no staleness
@@ -126,11 +127,27 @@ def build_cpp_suites() -> list[str]:
         return list(ex.map(one, CPP_SUITES))
 
 
+CLI_DIR = os.path.join(PKG, "cli")
+
+
+def build_cli() -> str:
+    """bin/LatticeBoltzmannHip: the reference's command-line driver on the
+    facade."""
+    os.makedirs(BINDIR, exist_ok=True)
+    out = os.path.join(BINDIR, "LatticeBoltzmannHip")
+    _run([CXX, *CXX_FLAGS, os.path.join(CLI_DIR, "LatticeBoltzmannHip.cpp"),
+          f"-L{LIBDIR}", "-llbm", "-llbm_hip", "-Wl,-rpath,$ORIGIN/../lib",
+          "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all() -> list[str]:
     if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     out = [build_hip(), build_facade()]
     out += build_cpp_suites()
+    out.append(build_cli())
     return out
 
 

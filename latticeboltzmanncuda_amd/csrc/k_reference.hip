@@ -10,8 +10,10 @@
 //   k_ref_stream_collide  Lattice.cpp:222-226     both of the above fused
 //   k_ref_reflect         SimCalcFuncs.cu:158-198 reflection tables
 //   k_ref_flow            SimCalcFuncs.cu:200-215 flow injection
-//   k_ref_step_full       LatticeBoltzmannCuda.cpp:85-93, the main loop's
-//                         stream + reflect + collide + flow in one pass
+//   k_ref_reflect_gather  the reflections of one step as a gather over
+//   k_ref_step_full       (receiver, direction) groups, then the main
+//                         loop's stream + reflect + collide + flow in one
+//                         pass (LatticeBoltzmannCuda.cpp:85-93)
 //
 // The reference copies the whole lattice before streaming and before
 // reflecting (Lattice.cpp:126-133,176-183).  Here streaming goes A -> B and
@@ -239,94 +241,90 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// Gather pass of the fused reference step: one thread per group (t, j).
+// The group's value is the staged pipeline's f_j(t) after reflection:
+// the streamed value A_j(x) of the reflecting node x = t - e_j, then, in
+// ascending slot order, + W[s] * F_k(x) with F_k(x) = A_k(x - e_k) (0
+// outside the box) -- the same additions in the same order as the staged
+// pass.  The force gets val * (e_j SPLIT_j - e_s SPLIT_s) per slot
+// (SimCalcFuncs.cu:181-183), summed per wavefront.
+__global__ void __launch_bounds__(kBlock)
+    k_ref_reflect_gather(const float* __restrict__ A, uint32_t ngrp,
+                         const uint32_t* __restrict__ gsrc,
+                         const uint32_t* __restrict__ gbeg, uint32_t nent,
+                         const uint32_t* __restrict__ eidx,
+                         const float* __restrict__ ew,
+                         const float4* __restrict__ ef, float* __restrict__ gsum,
+                         float* __restrict__ partials) {
+  const uint32_t gi = blockIdx.x * kBlock + threadIdx.x;
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  if (gi < ngrp) {
+    float acc = A[gsrc[gi]];
+    const uint32_t end = gi + 1 < ngrp ? gbeg[gi + 1] : nent;
+    for (uint32_t e = gbeg[gi]; e < end; ++e) {
+      const uint32_t el = eidx[e];
+      const float v = el != ~0u ? A[el] : 0.0f;
+      const float val = v * ew[e];
+      acc = acc + val;
+      const float4 cf = ef[e];
+      fx += val * cf.x;
+      fy += val * cf.y;
+      fz += val * cf.z;
+    }
+    gsum[gi] = acc;
+  }
+  if (partials) {
+    const float sx = wave_sum(fx), sy = wave_sum(fy), sz = wave_sum(fz);
+    if ((threadIdx.x & 63) == 0) {
+      const uint32_t w = gi >> 6;
+      partials[3 * w] = sx;
+      partials[3 * w + 1] = sy;
+      partials[3 * w + 2] = sz;
+    }
+  }
+}
+
 // One step of the reference's main loop in one pass (A -> B):
 //   S  f_j(n) = A_j(n - e_j), 0 outside the box;
-//   R  internal nodes are zeroed; a receiving node n adds, for each
-//      incoming direction j (bit j of its in-mask) and each slot s of the
-//      reflecting node x = n - e_j that targets (n, j), in ascending s:
-//      W[s] * F_{s/3}(x), where F_k(x) = A_k(x - e_k) is x's streamed value
-//      (the second hop of the gather); the force gets the same terms as the
-//      staged pass (SimCalcFuncs.cu:181-183);
+//   R  internal nodes are zeroed; for a receiving node n the gather pass
+//      has already formed f_j(n) for every incoming direction j (bit j of
+//      its in-mask; its groups ascend in j from rxbase[n]);
 //   C  collide27; F flow injection.
-// Every target (n, j) has one possible source x, so the additions happen in
-// the order of the staged pass and the populations are bit-identical to
-// it; only the force's partial sums are grouped differently (by receiving
-// node instead of reflecting node).
+// The populations are bit-identical to the staged passes.
 template <bool GEOM>
 __global__ void __launch_bounds__(kBlock)
     k_ref_step_full(Grid g, const float* __restrict__ A, float* __restrict__ B,
                     float tau, FlowSpec flow, const uint32_t* __restrict__ cls,
                     const uint32_t* __restrict__ inmask,
                     const int32_t* __restrict__ rxbase,
-                    const uint32_t* __restrict__ rxent,
-                    const float* __restrict__ rxw, float* __restrict__ partials) {
+                    const float* __restrict__ gsum) {
   const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-  if (n < g.n) {
-    uint32_t ux, uy, uz;
-    decode(g, n, ux, uy, uz);
-    const int x = (int)ux, y = (int)uy, z = (int)uz;
-    float f[27];
-    pull27(g, A, n, x, y, z, f);
-    if (GEOM) {
-      const uint32_t c = (cls[n >> 4] >> (2u * (n & 15u))) & 3u;
-      if (c == 2u) {
+  if (n >= g.n) return;
+  uint32_t ux, uy, uz;
+  decode(g, n, ux, uy, uz);
+  const int x = (int)ux, y = (int)uy, z = (int)uz;
+  float f[27];
+  pull27(g, A, n, x, y, z, f);
+  if (GEOM) {
+    const uint32_t c = (cls[n >> 4] >> (2u * (n & 15u))) & 3u;
+    if (c == 2u) {
 #pragma unroll
-        for (int i = 0; i < 27; ++i) f[i] = 0.0f;
-      } else if (c == 1u) {
-        const uint32_t m = inmask[n];
-        uint32_t p = (uint32_t)rxbase[n];
+      for (int i = 0; i < 27; ++i) f[i] = 0.0f;
+    } else if (c == 1u) {
+      const uint32_t m = inmask[n];
+      uint32_t p = (uint32_t)rxbase[n];
 #pragma unroll
-        for (int j = 1; j < 27; ++j) {
-          if ((m >> j) & 1u) {
-            // the reflecting node x = n - e_j (inside the box by construction)
-            const int xs = x - L27::E[j][0], ys = y - L27::E[j][1], zs = z - L27::E[j][2];
-            const uint32_t cnt = rxent[p++];
-            for (uint32_t t = 0; t < cnt; ++t, ++p) {
-              const uint32_t e = rxent[p];
-              const int k = (int)(e & 31u), sl = (int)(e >> 5);
-              const int sx = xs - L27::E[k][0], sy = ys - L27::E[k][1], sz = zs - L27::E[k][2];
-              const bool in = sx >= 0 && sy >= 0 && sz >= 0 && sx < (int)g.nx &&
-                              sy < (int)g.ny && sz < (int)g.nz;
-              const uint32_t src = in ? (uint32_t)sz + g.nz * ((uint32_t)sy + g.ny * (uint32_t)sx) : n;
-              const float v = in ? A[pidx<27>(src, k)] : 0.0f;
-              const float val = v * rxw[p];
-              f[j] = f[j] + val;
-              const float ts0 = sl < 27 ? (float)L27::E[sl][0] * kSplit27[sl] : 0.0f;
-              const float ts1 = sl < 27 ? (float)L27::E[sl][1] * kSplit27[sl] : 0.0f;
-              const float ts2 = sl < 27 ? (float)L27::E[sl][2] * kSplit27[sl] : 0.0f;
-              fx += val * (((float)L27::E[j][0] * kSplit27[j]) - ts0);
-              fy += val * (((float)L27::E[j][1] * kSplit27[j]) - ts1);
-              fz += val * (((float)L27::E[j][2] * kSplit27[j]) - ts2);
-            }
-          }
-        }
-      }
-    }
-    collide27(f, tau);
-    if (flow.kind != LBM_FLOW_NONE && flow_match(flow, n, x, y, z)) {
-#pragma unroll
-      for (int i = 0; i < 27; ++i) f[i] = f[i] + flow.induced[i];
-    }
-#pragma unroll
-    for (int i = 0; i < 27; ++i) B[pidx<27>(n, i)] = f[i];
-  }
-  if (GEOM && partials) {
-    __shared__ float red[3][kBlock / 64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const float sx = wave_sum(fx), sy = wave_sum(fy), sz = wave_sum(fz);
-    if (lane == 0) {
-      red[0][wv] = sx;
-      red[1][wv] = sy;
-      red[2][wv] = sz;
-    }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-      float s = 0.0f;
-      for (int w = 0; w < kBlock / 64; ++w) s += red[threadIdx.x][w];
-      partials[3 * blockIdx.x + threadIdx.x] = s;
+      for (int j = 1; j < 27; ++j)
+        if ((m >> j) & 1u) f[j] = gsum[p++];
     }
   }
+  collide27(f, tau);
+  if (flow.kind != LBM_FLOW_NONE && flow_match(flow, n, x, y, z)) {
+#pragma unroll
+    for (int i = 0; i < 27; ++i) f[i] = f[i] + flow.induced[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 27; ++i) B[pidx<27>(n, i)] = f[i];
 }
 
 // Adds the per-block partial forces of one reflection pass, summed in FP64
@@ -439,27 +437,36 @@ int launch_ref_reflect(const Grid& g, float* f, int32_t nspecial,
   return LBM_OK;
 }
 
-int ref_full_blocks(uint32_t n) { return (int)blocks_for(n); }
+int ref_gather_partials(int64_t ngrp) {
+  return (int)(blocks_for((uint32_t)ngrp) * (kBlock / 64));
+}
 
 int launch_ref_step_full(const Grid& g, const float* A, float* B, float tau,
                          const FlowSpec& flow, const uint32_t* cls,
                          const uint32_t* inmask, const int32_t* rxbase,
-                         const uint32_t* rxent, const float* rxw,
-                         float* partials, float* force, hipStream_t s) {
+                         int64_t ngrp, const uint32_t* gsrc, const uint32_t* gbeg,
+                         float* gsum, int64_t nent, const uint32_t* eidx,
+                         const float* ew, const float4* ef, float* partials,
+                         float* force, hipStream_t s) {
   const unsigned nb = blocks_for(g.n);
-  if (cls) {
-    hipLaunchKernelGGL(k_ref_step_full<true>, dim3(nb), dim3(kBlock), 0, s, g, A, B, tau, flow,
-                       cls, inmask, rxbase, rxent, rxw, force ? partials : nullptr);
-  } else {
-    hipLaunchKernelGGL(k_ref_step_full<false>, dim3(nb), dim3(kBlock), 0, s, g, A, B, tau, flow,
-                       nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
-  }
-  LBM_HIP(hipGetLastError());
-  if (cls && force && partials) {
-    hipLaunchKernelGGL(k_ref_force_sum, dim3(1), dim3(kForceThreads), 0, s, partials, (int)nb,
-                       force);
+  if (cls && ngrp > 0) {
+    hipLaunchKernelGGL(k_ref_reflect_gather, dim3(blocks_for((uint32_t)ngrp)), dim3(kBlock), 0, s,
+                       A, (uint32_t)ngrp, gsrc, gbeg, (uint32_t)nent, eidx, ew, ef, gsum,
+                       force ? partials : nullptr);
     LBM_HIP(hipGetLastError());
+    if (force && partials) {
+      hipLaunchKernelGGL(k_ref_force_sum, dim3(1), dim3(kForceThreads), 0, s, partials,
+                         ref_gather_partials(ngrp), force);
+      LBM_HIP(hipGetLastError());
+    }
   }
+  if (cls)
+    hipLaunchKernelGGL(k_ref_step_full<true>, dim3(nb), dim3(kBlock), 0, s, g, A, B, tau, flow,
+                       cls, inmask, rxbase, gsum);
+  else
+    hipLaunchKernelGGL(k_ref_step_full<false>, dim3(nb), dim3(kBlock), 0, s, g, A, B, tau, flow,
+                       nullptr, nullptr, nullptr, nullptr);
+  LBM_HIP(hipGetLastError());
   return LBM_OK;
 }
 

@@ -208,14 +208,27 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-__global__ void k_sum_partials(const double* __restrict__ partials,
-                               int nblocks, double* __restrict__ out) {
-  // one wave; lane k < 4 sums component k in block order
-  const int k = threadIdx.x;
-  if (k >= 4) return;
-  double s = 0;
-  for (int b = 0; b < nblocks; ++b) s += partials[4 * b + k];
-  out[k] = s;
+// Sums the per-block FP64 partials (4 per block) in a fixed order: thread
+// t takes blocks t, t + 1024, ..., then a fixed tree -- the same result on
+// every run, and 1024 loads in flight instead of one chain.
+constexpr int kSumThreads = 1024;
+
+__global__ void __launch_bounds__(kSumThreads)
+    k_sum_partials(const double* __restrict__ partials, int nblocks,
+                   double* __restrict__ out) {
+  __shared__ double red[4][kSumThreads];
+  const int t = threadIdx.x;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = t; b < nblocks; b += kSumThreads)
+    for (int k = 0; k < 4; ++k) a[k] += partials[4 * b + k];
+  for (int k = 0; k < 4; ++k) red[k][t] = a[k];
+  __syncthreads();
+  for (int h = kSumThreads / 2; h > 0; h >>= 1) {
+    if (t < h)
+      for (int k = 0; k < 4; ++k) red[k][t] += red[k][t + h];
+    __syncthreads();
+  }
+  if (t < 4) out[t] = red[t][0];
 }
 
 template <int Q, int S>
@@ -375,7 +388,7 @@ int launch_conservation(int q, int storage, const Grid& g, const void* f,
   });
   if (drc) return drc;
   LBM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, partials,
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kSumThreads), 0, s, partials,
                      nblocks, out4);
   LBM_HIP(hipGetLastError());
   return LBM_OK;

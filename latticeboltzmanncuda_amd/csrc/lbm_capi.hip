@@ -222,21 +222,26 @@ int validate_config(const lbm_config* cfg) {
 
 // Tables of the fused reference step.  Every slot s (1..80) of a
 // reflecting node x with direction d (1..26) whose target t = x + e_d lies
-// in the box becomes an entry of receiver t for direction d; entries of one
-// (t, d) keep ascending s, the order in which the staged reflection pass
-// adds them.  Targets must not be flagged: the staged pass would otherwise
-// race between adding to a target and zeroing it.
+// in the box becomes an entry of group (t, d); a group's entries keep
+// ascending s, the order in which the staged reflection pass adds them, and
+// groups are ordered by (t, d).  Targets must not be flagged: the staged
+// pass would otherwise race between adding to a target and zeroing it.
 int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
   const Grid& g = c->grid;
   const int64_t N = g.n;
   const size_t words = (size_t)((N + 15) / 16);
+  const size_t ng = (size_t)std::max<int64_t>(c->ref_ngrp, 1), ne = (size_t)std::max<int64_t>(c->ref_nent, 1);
   dev_free(c, (void**)&c->ref_cls, sizeof(uint32_t) * words);
   dev_free(c, (void**)&c->ref_inmask, sizeof(uint32_t) * (size_t)N);
   dev_free(c, (void**)&c->ref_rxbase, sizeof(int32_t) * (size_t)N);
-  dev_free(c, (void**)&c->ref_rxent, sizeof(uint32_t) * (size_t)std::max<int64_t>(c->ref_nent, 1));
-  dev_free(c, (void**)&c->ref_rxw, sizeof(float) * (size_t)std::max<int64_t>(c->ref_nent, 1));
+  dev_free(c, (void**)&c->ref_gsrc, sizeof(uint32_t) * ng);
+  dev_free(c, (void**)&c->ref_gbeg, sizeof(uint32_t) * ng);
+  dev_free(c, (void**)&c->ref_gsum, sizeof(float) * ng);
+  dev_free(c, (void**)&c->ref_eidx, sizeof(uint32_t) * ne);
+  dev_free(c, (void**)&c->ref_ew, sizeof(float) * ne);
+  dev_free(c, (void**)&c->ref_ef, sizeof(float4) * ne);
   dev_free(c, (void**)&c->ref_partials, sizeof(float) * 3 * (size_t)std::max(c->ref_partials_cap, 1));
-  c->ref_nent = 0;
+  c->ref_ngrp = c->ref_nent = 0;
   c->ref_partials_cap = 0;
   c->ref_geom = c->nspecial > 0;
   if (!c->ref_geom) return LBM_OK;
@@ -247,6 +252,8 @@ int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
     uint8_t s;    // slot of the reflecting node's record
     int32_t r;    // record
   };
+  const int64_t plane = (int64_t)g.ny * g.nz;
+  auto at = [&](int64_t x, int64_t y, int64_t z) { return z + (int64_t)g.nz * (y + (int64_t)g.ny * x); };
   std::vector<Link> links;
   std::vector<uint32_t> cls(words, 0u);
   for (int64_t n = 0; n < N; ++n) {
@@ -254,15 +261,14 @@ int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
     if (fl & LBM_FLAG_INTERNAL) cls[n >> 4] |= 2u << (2 * (n & 15));
     const int32_t r = c->nrec > 0 ? rec_of[n] : -1;
     if (!(fl & LBM_FLAG_REFLECTED) || r < 0) continue;
-    const int64_t x = n / ((int64_t)g.ny * g.nz), rem = n % ((int64_t)g.ny * g.nz);
-    const int64_t y = rem / g.nz, z = rem % g.nz;
+    const int64_t x = n / plane, y = (n % plane) / g.nz, z = n % g.nz;
     const uint32_t* w = c->host_rec_dirs.data() + 14 * (size_t)r;
     for (int s = 1; s < 81; ++s) {
       const int d = (int)((w[s / 6] >> (5 * (s % 6))) & 0x1Fu);
       if (d == 0 || d > 26) continue;
       const int64_t tx = x + Lat<27>::E[d][0], ty = y + Lat<27>::E[d][1], tz = z + Lat<27>::E[d][2];
       if (tx < 0 || ty < 0 || tz < 0 || tx >= g.nx || ty >= g.ny || tz >= g.nz) continue;
-      const int64_t t = tz + (int64_t)g.nz * (ty + (int64_t)g.ny * tx);
+      const int64_t t = at(tx, ty, tz);
       if (c->host_flags[t])
         return fail(LBM_ERR_INVALID, "reflection target " + std::to_string(t) +
                                          " is a flagged node (targets must be fluid)");
@@ -272,46 +278,70 @@ int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
   std::stable_sort(links.begin(), links.end(), [](const Link& a, const Link& b) {
     return a.t != b.t ? a.t < b.t : (a.d != b.d ? a.d < b.d : a.s < b.s);
   });
-  std::vector<uint32_t> inmask((size_t)N, 0u), ent;
+  // SimCalcFuncs.cu:181-183: the force term of slot s sent in direction d
+  // is val * (e_d SPLIT_d - e_s SPLIT_s), the second factor taken as 0 for
+  // s >= 27 (outside the reference's tables)
+  auto esplit = [](int i, int a) { return (float)Lat<27>::E[i][a] * kSplit27[i]; };
+  std::vector<uint32_t> inmask((size_t)N, 0u), gsrc, gbeg, eidx;
   std::vector<int32_t> base((size_t)N, -1);
-  std::vector<float> wts;
+  std::vector<float> ew;
+  std::vector<float4> ef;
   for (size_t i = 0; i < links.size();) {
     const uint32_t t = links[i].t;
-    base[t] = (int32_t)ent.size();
+    base[t] = (int32_t)gsrc.size();
     cls[t >> 4] |= 1u << (2 * (t & 15));
+    const int64_t tx = t / plane, ty = (t % plane) / g.nz, tz = t % g.nz;
     while (i < links.size() && links[i].t == t) {
-      const uint8_t d = links[i].d;
+      const int d = links[i].d;
       inmask[t] |= 1u << d;
-      size_t j = i;
-      while (j < links.size() && links[j].t == t && links[j].d == d) ++j;
-      ent.push_back((uint32_t)(j - i));
-      wts.push_back(0.0f);
-      for (size_t k = i; k < j; ++k) {
-        const int s = links[k].s;
-        ent.push_back((uint32_t)(s / 3) | ((uint32_t)s << 5));
-        wts.push_back(c->host_rec_w[81 * (size_t)links[k].r + (size_t)s]);
+      // the reflecting node x = t - e_d (inside the box: it holds a record)
+      const int64_t xx = tx - Lat<27>::E[d][0], xy = ty - Lat<27>::E[d][1], xz = tz - Lat<27>::E[d][2];
+      gsrc.push_back(pidx<27>((uint32_t)at(xx, xy, xz), d));
+      gbeg.push_back((uint32_t)eidx.size());
+      for (; i < links.size() && links[i].t == t && links[i].d == d; ++i) {
+        const int s = links[i].s, k = s / 3;
+        // x's streamed value of direction k: A_k(x - e_k), 0 outside
+        const int64_t sx = xx - Lat<27>::E[k][0], sy = xy - Lat<27>::E[k][1], sz = xz - Lat<27>::E[k][2];
+        const bool in = sx >= 0 && sy >= 0 && sz >= 0 && sx < g.nx && sy < g.ny && sz < g.nz;
+        eidx.push_back(in ? pidx<27>((uint32_t)at(sx, sy, sz), k) : ~0u);
+        ew.push_back(c->host_rec_w[81 * (size_t)links[i].r + (size_t)s]);
+        float4 fc;
+        fc.x = esplit(d, 0) - (s < 27 ? esplit(s, 0) : 0.0f);
+        fc.y = esplit(d, 1) - (s < 27 ? esplit(s, 1) : 0.0f);
+        fc.z = esplit(d, 2) - (s < 27 ? esplit(s, 2) : 0.0f);
+        fc.w = 0.0f;
+        ef.push_back(fc);
       }
-      i = j;
     }
   }
-  if (ent.size() >= (1ull << 31)) return fail(LBM_ERR_UNSUPPORTED, "too many reflected links");
-  c->ref_nent = (int64_t)ent.size();
-  c->ref_partials_cap = ref_full_blocks(g.n);
+  if (eidx.size() >= (1ull << 31)) return fail(LBM_ERR_UNSUPPORTED, "too many reflected links");
+  c->ref_ngrp = (int64_t)gsrc.size();
+  c->ref_nent = (int64_t)eidx.size();
+  c->ref_partials_cap = ref_gather_partials(c->ref_ngrp);
+  const size_t ng2 = std::max<size_t>(gsrc.size(), 1), ne2 = std::max<size_t>(eidx.size(), 1);
   int rc;
   if ((rc = dev_alloc(c, (void**)&c->ref_cls, sizeof(uint32_t) * words, "hipMalloc(ref classes)")) ||
       (rc = dev_alloc(c, (void**)&c->ref_inmask, sizeof(uint32_t) * (size_t)N, "hipMalloc(ref in-masks)")) ||
       (rc = dev_alloc(c, (void**)&c->ref_rxbase, sizeof(int32_t) * (size_t)N, "hipMalloc(ref receivers)")) ||
-      (rc = dev_alloc(c, (void**)&c->ref_rxent, sizeof(uint32_t) * std::max<size_t>(ent.size(), 1), "hipMalloc(ref entries)")) ||
-      (rc = dev_alloc(c, (void**)&c->ref_rxw, sizeof(float) * std::max<size_t>(wts.size(), 1), "hipMalloc(ref weights)")) ||
-      (rc = dev_alloc(c, (void**)&c->ref_partials, sizeof(float) * 3 * (size_t)c->ref_partials_cap, "hipMalloc(ref force partials)")))
+      (rc = dev_alloc(c, (void**)&c->ref_gsrc, sizeof(uint32_t) * ng2, "hipMalloc(ref groups)")) ||
+      (rc = dev_alloc(c, (void**)&c->ref_gbeg, sizeof(uint32_t) * ng2, "hipMalloc(ref groups)")) ||
+      (rc = dev_alloc(c, (void**)&c->ref_gsum, sizeof(float) * ng2, "hipMalloc(ref group sums)")) ||
+      (rc = dev_alloc(c, (void**)&c->ref_eidx, sizeof(uint32_t) * ne2, "hipMalloc(ref entries)")) ||
+      (rc = dev_alloc(c, (void**)&c->ref_ew, sizeof(float) * ne2, "hipMalloc(ref weights)")) ||
+      (rc = dev_alloc(c, (void**)&c->ref_ef, sizeof(float4) * ne2, "hipMalloc(ref force terms)")) ||
+      (rc = dev_alloc(c, (void**)&c->ref_partials, sizeof(float) * 3 * (size_t)std::max(c->ref_partials_cap, 1),
+                      "hipMalloc(ref force partials)")))
     return rc;
   if ((rc = h2d(c, c->ref_cls, cls.data(), sizeof(uint32_t) * words)) ||
       (rc = h2d(c, c->ref_inmask, inmask.data(), sizeof(uint32_t) * (size_t)N)) ||
       (rc = h2d(c, c->ref_rxbase, base.data(), sizeof(int32_t) * (size_t)N)))
     return rc;
-  if (!ent.empty() &&
-      ((rc = h2d(c, c->ref_rxent, ent.data(), sizeof(uint32_t) * ent.size())) ||
-       (rc = h2d(c, c->ref_rxw, wts.data(), sizeof(float) * wts.size()))))
+  if (!gsrc.empty() &&
+      ((rc = h2d(c, c->ref_gsrc, gsrc.data(), sizeof(uint32_t) * gsrc.size())) ||
+       (rc = h2d(c, c->ref_gbeg, gbeg.data(), sizeof(uint32_t) * gbeg.size())) ||
+       (rc = h2d(c, c->ref_eidx, eidx.data(), sizeof(uint32_t) * eidx.size())) ||
+       (rc = h2d(c, c->ref_ew, ew.data(), sizeof(float) * ew.size())) ||
+       (rc = h2d(c, c->ref_ef, ef.data(), sizeof(float4) * ef.size()))))
     return rc;
   return LBM_OK;
 }
@@ -408,8 +438,9 @@ int step_reference_full(lbm_ctx* c, int n, int with_force) {
   for (int s = 0; s < n; ++s) {
     if ((rc = launch_ref_step_full(c->grid, (const float*)c->f[c->cur], (float*)c->f[c->cur ^ 1],
                                    c->cfg.tau, c->flow, geom ? c->ref_cls : nullptr,
-                                   c->ref_inmask, c->ref_rxbase, c->ref_rxent, c->ref_rxw,
-                                   geom && with_force ? c->ref_partials : nullptr,
+                                   c->ref_inmask, c->ref_rxbase, c->ref_ngrp, c->ref_gsrc,
+                                   c->ref_gbeg, c->ref_gsum, c->ref_nent, c->ref_eidx, c->ref_ew,
+                                   c->ref_ef, geom && with_force ? c->ref_partials : nullptr,
                                    geom && with_force ? c->force : nullptr, c->stream)))
       return rc;
     c->cur ^= 1;
@@ -542,7 +573,9 @@ int lbm_destroy(lbm_ctx* c) {
                   (void*)c->flow_mask, (void*)c->force,
                   (void*)c->force_partials, (void*)c->red, (void*)c->red_out,
                   (void*)c->ref_cls, (void*)c->ref_inmask, (void*)c->ref_rxbase,
-                  (void*)c->ref_rxent, (void*)c->ref_rxw, (void*)c->ref_partials})
+                  (void*)c->ref_gsrc, (void*)c->ref_gbeg, (void*)c->ref_gsum,
+                  (void*)c->ref_eidx, (void*)c->ref_ew, (void*)c->ref_ef,
+                  (void*)c->ref_partials})
     if (p) (void)hipFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);

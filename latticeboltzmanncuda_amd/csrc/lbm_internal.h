@@ -123,15 +123,18 @@ int launch_ref_reflect(const Grid& g, float* f, int32_t nspecial,
                        const float* rec_w, float* force_partials,
                        int nblocks, float* force, hipStream_t s);
 int ref_reflect_blocks(int32_t nspecial);
-// Fused stream + reflect + collide + flow (A -> B); geometry tables may be
-// null when no node is flagged; partials (3 per block, ref_full_blocks) and
-// force may be null to skip the force.
+// Fused stream + reflect + collide + flow (A -> B).  With reflections
+// (cls non-null) a gather pass over the (receiver, direction) groups runs
+// first; partials (3 per gather wavefront, ref_gather_partials) and force
+// may be null to skip the force.
 int launch_ref_step_full(const Grid& g, const float* A, float* B, float tau,
                          const FlowSpec& flow, const uint32_t* cls,
                          const uint32_t* inmask, const int32_t* rxbase,
-                         const uint32_t* rxent, const float* rxw,
-                         float* partials, float* force, hipStream_t s);
-int ref_full_blocks(uint32_t n);
+                         int64_t ngrp, const uint32_t* gsrc, const uint32_t* gbeg,
+                         float* gsum, int64_t nent, const uint32_t* eidx,
+                         const float* ew, const float4* ef, float* partials,
+                         float* force, hipStream_t s);
+int ref_gather_partials(int64_t ngrp);
 int launch_ref_flow(const Grid& g, float* f, const FlowSpec& flow,
                     hipStream_t s);
 int launch_ref_moments(const Grid& g, const float* f, float* rho, float* u,
@@ -189,16 +192,23 @@ struct lbm_ctx {
   int force_partials_cap = 0;
 
   // fused reference step (stream + reflect + collide + flow in one pass):
-  // the nodes that receive reflected populations gather them from the
-  // reflecting node's streamed values
+  // a gather pass sums, for every (receiving node t, direction j), the
+  // streamed value A_j(x) of the reflecting node x = t - e_j and the
+  // reflected slot values W[s] F_{s/3}(x) in ascending s; the fused pass
+  // then takes that sum as f_j(t)
   bool ref_geom = false;           // any flagged node
   uint32_t* ref_cls = nullptr;     // 2-bit: 0 plain, 1 receiver, 2 internal
   uint32_t* ref_inmask = nullptr;  // per node: bit j = a reflected link arrives in direction j
-  int32_t* ref_rxbase = nullptr;   // per node: first entry of a receiver
-  uint32_t* ref_rxent = nullptr;   // per receiver, per incoming j: count, then (k | s << 5)
-  float* ref_rxw = nullptr;        // slot weights, parallel to ref_rxent
-  int64_t ref_nent = 0;
-  float* ref_partials = nullptr;   // force partials of the fused step, 3 per block
+  int32_t* ref_rxbase = nullptr;   // per node: first group of a receiver (groups ascend in j)
+  int64_t ref_ngrp = 0;            // groups (t, j)
+  uint32_t* ref_gsrc = nullptr;    // per group: element index of A_j(x)
+  uint32_t* ref_gbeg = nullptr;    // per group: first entry; the last group ends at ref_nent
+  float* ref_gsum = nullptr;       // per group: the gathered f_j(t) of the current step
+  int64_t ref_nent = 0;            // entries (one per reflected slot)
+  uint32_t* ref_eidx = nullptr;    // per entry: element index of F_k(x) = A_k(x - e_k), ~0u outside
+  float* ref_ew = nullptr;         // per entry: slot weight W[s]
+  float4* ref_ef = nullptr;        // per entry: force coefficients (x, y, z, unused)
+  float* ref_partials = nullptr;   // force partials of the gather pass, 3 per wavefront
   int ref_partials_cap = 0;
   double* red = nullptr;
   int red_cap = 0;

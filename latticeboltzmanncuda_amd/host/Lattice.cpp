@@ -119,6 +119,11 @@ void Lattice::simulateFlow() {
 
 void Lattice::simulateLattice() { lbm::check(lbm_step(m_ctx, 1), "lbm_step"); }
 
+void Lattice::simulateStep() {
+  lbm::check(lbm_step_reference(m_ctx, 1, m_reflectionData != nullptr ? 1 : 0),
+             "lbm_step_reference");
+}
+
 std::vector<uint8_t> Lattice::downloadFlags() {
   std::vector<uint8_t> flags((std::size_t)m_xResolution * m_yResolution * m_zResolution);
   lbm::check(lbm_get_flags(m_ctx, flags.data()), "lbm_get_flags");

@@ -81,15 +81,48 @@ __device__ __forceinline__ void pull27(const Grid& g, const float* __restrict__ 
   }
 }
 
-// SimCalcFuncs.cu:102-155 on one node's 27 populations.
-__device__ __forceinline__ void collide27(float f[27], float tau) {
+// The relaxation time and its reciprocal rounded to double (computed on
+// the host); see div_tau.
+struct Relax {
+  float tau;
+  double rinv;
+};
+
+inline Relax make_relax(float tau) { return Relax{tau, 1.0 / (double)tau}; }
+
+// x / tau rounded to float, as the reference's IEEE division gives it.
+// q = x * RN64(1/tau) is within 2^-52 (relative) of the exact quotient,
+// while the quotient of two floats lies at least 2^-49 away from any point
+// halfway between two floats (it can never be one: a halfway point needs
+// 25 significant bits, and x = midpoint * tau would then need more than
+// 24), so rounding q to float gives the correctly rounded quotient.  Below
+// the float normal range the rounding position moves, so tiny quotients
+// take the IEEE division itself.
+__device__ __forceinline__ float div_tau(float x, const Relax& r) {
+  const double q = (double)x * r.rinv;
+  if (fabs(q) < 1.1754943508222875e-38) return x / r.tau;
+  return (float)q;
+}
+
+// SimCalcFuncs.cu:102-155 on one node's 27 populations, bit-identical to
+// evaluating the reference source as written, with fewer instructions:
+//  - terms e_i,a * u_a with e_i,a = 0 are left out of the velocity sums and
+//    of e.u: for finite values they only add a signed zero, and the sign of
+//    a zero e.u or u_a cannot reach the result (u_a and e.u enter only
+//    through squares and 1 + 3 e.u);
+//  - 3.0 * ct, 4.5 * ct and (4.5 * ct) * ct, 1.5 * usq are exact in double
+//    (ct, usq are floats: at most 2 + 24, 4 + 24, 28 + 24 and 2 + 24
+//    significant bits), so each product-then-add is one FMA with the same
+//    single rounding; the additions keep the source's order;
+//  - the division by tau is div_tau.
+__device__ __forceinline__ void collide27(float f[27], const Relax& r) {
   float density = 0.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f;
 #pragma unroll
   for (int i = 0; i < 27; ++i) {
     density += f[i];
-    ux += f[i] * (float)L27::E[i][0] * kSplit27[i];
-    uy += f[i] * (float)L27::E[i][1] * kSplit27[i];
-    uz += f[i] * (float)L27::E[i][2] * kSplit27[i];
+    if (L27::E[i][0]) ux += (L27::E[i][0] > 0 ? f[i] : -f[i]) * kSplit27[i];
+    if (L27::E[i][1]) uy += (L27::E[i][1] > 0 ? f[i] : -f[i]) * kSplit27[i];
+    if (L27::E[i][2]) uz += (L27::E[i][2] > 0 ? f[i] : -f[i]) * kSplit27[i];
   }
   if (density != 0.0f) {
     ux /= density;
@@ -97,15 +130,33 @@ __device__ __forceinline__ void collide27(float f[27], float tau) {
     uz /= density;
   }
   const float usq = (ux * ux) + (uy * uy) + (uz * uz);
+  const double m15usq = -1.5 * (double)usq;
 #pragma unroll
   for (int i = 0; i < 27; ++i) {
-    const float ct = ((float)L27::E[i][0] * ux) + ((float)L27::E[i][1] * uy) +
-                     ((float)L27::E[i][2] * uz);
+    float ct = 0.0f;
+    bool first = true;
+    if (L27::E[i][0]) {
+      ct = L27::E[i][0] > 0 ? ux : -ux;
+      first = false;
+    }
+    if (L27::E[i][1]) {
+      const float t = L27::E[i][1] > 0 ? uy : -uy;
+      ct = first ? t : ct + t;
+      first = false;
+    }
+    if (L27::E[i][2]) {
+      const float t = L27::E[i][2] > 0 ? uz : -uz;
+      ct = first ? t : ct + t;
+    }
     const double w = i == 0 ? 8.0 / 27.0
                             : (i < 7 ? 2.0 / 27.0 : (i < 19 ? 1.0 / 54.0 : 1.0 / 216.0));
-    const float eq =
-        (float)(w * density * (1 + (3.0 * ct) + (4.5 * ct * ct) - (1.5 * usq)));
-    f[i] = f[i] + (eq - f[i]) / tau;
+    const double c = (double)ct;
+    // ((1 + 3 ct) + (4.5 ct) ct) - 1.5 usq, each step rounded once
+    const double t1 = fma(3.0, c, 1.0);
+    const double t2 = fma(4.5 * c, c, t1);
+    const double t3 = t2 + m15usq;
+    const float eq = (float)((w * (double)density) * t3);
+    f[i] = f[i] + div_tau(eq - f[i], r);
   }
 }
 
@@ -122,7 +173,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 __global__ void __launch_bounds__(kBlock)
-    k_ref_collide(Grid g, float* __restrict__ F, float tau, FlowSpec flow) {
+    k_ref_collide(Grid g, float* __restrict__ F, Relax tau, FlowSpec flow) {
   const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
   if (n >= g.n) return;
   float f[27];
@@ -143,7 +194,7 @@ __global__ void __launch_bounds__(kBlock)
 
 __global__ void __launch_bounds__(kBlock)
     k_ref_stream_collide(Grid g, const float* __restrict__ A,
-                         float* __restrict__ B, float tau) {
+                         float* __restrict__ B, Relax tau) {
   const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
   if (n >= g.n) return;
   uint32_t x, y, z;
@@ -294,7 +345,7 @@ __global__ void __launch_bounds__(kBlock)
 template <bool GEOM>
 __global__ void __launch_bounds__(kBlock)
     k_ref_step_full(Grid g, const float* __restrict__ A, float* __restrict__ B,
-                    float tau, FlowSpec flow, const uint32_t* __restrict__ cls,
+                    Relax tau, FlowSpec flow, const uint32_t* __restrict__ cls,
                     const uint32_t* __restrict__ inmask,
                     const int32_t* __restrict__ rxbase,
                     const float* __restrict__ gsum) {
@@ -402,7 +453,7 @@ int launch_ref_collide(const Grid& g, float* f, float tau,
   FlowSpec none{};
   none.kind = LBM_FLOW_NONE;
   hipLaunchKernelGGL(k_ref_collide, dim3(blocks_for(g.n)), dim3(kBlock), 0, s,
-                     g, f, tau, flow ? *flow : none);
+                     g, f, make_relax(tau), flow ? *flow : none);
   LBM_HIP(hipGetLastError());
   return LBM_OK;
 }
@@ -410,7 +461,7 @@ int launch_ref_collide(const Grid& g, float* f, float tau,
 int launch_ref_stream_collide(const Grid& g, const float* A, float* B,
                               float tau, hipStream_t s) {
   hipLaunchKernelGGL(k_ref_stream_collide, dim3(blocks_for(g.n)),
-                     dim3(kBlock), 0, s, g, A, B, tau);
+                     dim3(kBlock), 0, s, g, A, B, make_relax(tau));
   LBM_HIP(hipGetLastError());
   return LBM_OK;
 }
@@ -461,11 +512,11 @@ int launch_ref_step_full(const Grid& g, const float* A, float* B, float tau,
     }
   }
   if (cls)
-    hipLaunchKernelGGL(k_ref_step_full<true>, dim3(nb), dim3(kBlock), 0, s, g, A, B, tau, flow,
-                       cls, inmask, rxbase, gsum);
+    hipLaunchKernelGGL(k_ref_step_full<true>, dim3(nb), dim3(kBlock), 0, s, g, A, B,
+                       make_relax(tau), flow, cls, inmask, rxbase, gsum);
   else
-    hipLaunchKernelGGL(k_ref_step_full<false>, dim3(nb), dim3(kBlock), 0, s, g, A, B, tau, flow,
-                       nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_ref_step_full<false>, dim3(nb), dim3(kBlock), 0, s, g, A, B,
+                       make_relax(tau), flow, nullptr, nullptr, nullptr, nullptr);
   LBM_HIP(hipGetLastError());
   return LBM_OK;
 }

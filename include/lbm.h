@@ -333,6 +333,18 @@ typedef struct lbm_serve_stats {
 int lbm_serve_run(const lbm_serve_params* params, lbm_request* requests,
                   int32_t count, lbm_serve_stats* stats);
 
+/* ------------------------------------------------------------ geometry */
+/* Marks the nodes inside a closed triangle mesh, on the GPU: inside[n] = 1
+ * when node (x, y, z), at position (x, y, z) * spacing (formed in float),
+ * lies on a face or has generalized winding number |W| > 1/2 -- the
+ * decisions of the host loop it replaces (Lattice::insertModel,
+ * Lattice.cpp:228-260, over Model::isPointInsideModel, Model.cpp:51-63).
+ * vertices: 3 floats each; faces: 3 vertex indices each; inside: one byte
+ * per node on the host; count (may be NULL): the number of inside nodes. */
+int lbm_voxelize_mesh(lbm_ctx* ctx, const float* vertices, int32_t nv,
+                      const int32_t* faces, int32_t nf, float spacing,
+                      uint8_t* inside, int64_t* count);
+
 /* ------------------------------------------------------------ measurement */
 /* No counterpart in the reference: copies a `bytes`-sized device buffer
  * into another with 16-byte accesses, iters times, and reports the

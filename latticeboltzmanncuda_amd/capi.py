@@ -184,6 +184,7 @@ _SIGNATURES = {
     "lbm_batch_residuals": (C.c_int, [P, P, i32]),
     "lbm_serve_run": (C.c_int, [C.POINTER(ServeParams), C.POINTER(Request),
                                 i32, C.POINTER(ServeStats)]),
+    "lbm_voxelize_mesh": (C.c_int, [CTX, P, i32, P, i32, C.c_float, P, P]),
     "lbm_measure_copy_bandwidth": (C.c_int, [i64, C.c_int,
                                              C.POINTER(C.c_double)]),
     "lbm_random_spheres": (i64, [i32, i32, i32, C.c_uint64, C.c_double,
@@ -621,6 +622,19 @@ def serve(requests, storage=STORE_F16, budget_bytes=200 * 10 ** 9,
             res["u"] = r.pop("_u").reshape(3, -1)
         results.append(res)
     return stats, results
+
+
+def voxelize_mesh(ctx: "Context", vertices, faces, spacing: float):
+    """Inside map (one byte per node) of a closed triangle mesh, computed on
+    the GPU for the lattice of ctx."""
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1)
+    f = np.ascontiguousarray(faces, np.int32).reshape(-1)
+    out = np.zeros(ctx.n, np.uint8)
+    cnt = C.c_int64(0)
+    check(load().lbm_voxelize_mesh(ctx.h, _ptr(v), len(v) // 3, _ptr(f),
+                                   len(f) // 3, float(spacing), _ptr(out),
+                                   C.byref(cnt)), "lbm_voxelize_mesh")
+    return out
 
 
 def measure_copy_bandwidth(nbytes: int, iters: int = 20) -> float:

@@ -233,14 +233,30 @@ void build_reflections(Model& model, int nx, int ny, int nz, float h,
 
 }  // namespace lbm
 
-// Lattice.cpp:228-260
+// Lattice.cpp:228-260.  The inside test of every node runs on the GPU
+// (lbm_voxelize_mesh) with the decisions of Model::isPointInsideModel; the
+// host loop lbm::mark_internal remains for checks.
 void Lattice::insertModel(std::string filename) {
   m_simModel.reset(new Model());
   m_simModel->importModel(filename);
-  std::vector<uint8_t> flags = downloadFlags();
-  if (!lbm::mark_internal(*m_simModel, m_xResolution, m_yResolution,
-                          m_zResolution, m_latticeSpacing, flags))
+  if (!m_simModel->isLoaded()) return;
+  const MeshBounds b = m_simModel->bounding_box();
+  if (b.xmin() < 0.0f || b.ymin() < 0.0f || b.zmin() < 0.0f ||
+      b.xmax() > m_xResolution * m_latticeSpacing ||
+      b.ymax() > m_yResolution * m_latticeSpacing ||
+      b.zmax() > m_zResolution * m_latticeSpacing)
     return;
+  std::vector<uint8_t> flags = downloadFlags();
+  std::vector<uint8_t> inside(flags.size(), 0);
+  if (!m_simModel->checkError()) {  // an invalid mesh has no inside
+    const TriangleMesh& mesh = m_simModel->getMesh();
+    lbm::check(lbm_voxelize_mesh(m_ctx, mesh.vertices.data(), (int32_t)mesh.number_of_vertices(),
+                                 mesh.faces.data(), (int32_t)mesh.number_of_faces(),
+                                 m_latticeSpacing, inside.data(), nullptr),
+               "lbm_voxelize_mesh");
+  }
+  for (std::size_t n = 0; n < flags.size(); ++n)
+    flags[n] = (uint8_t)((flags[n] & ~LBM_FLAG_INTERNAL) | (inside[n] ? LBM_FLAG_INTERNAL : 0u));
   lbm::check(lbm_upload_flags(m_ctx, flags.data()), "lbm_upload_flags");
 }
 

@@ -1,0 +1,123 @@
+"""GPU mesh voxelisation (lbm_voxelize_mesh) against the oracle's
+point-in-mesh loop: identical inside maps for the reference tests' cubes
+and for a 1280-face sphere mesh, at several lattice sizes and spacings."""
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from latticeboltzmanncuda_amd import capi
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                    "dataFiles")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def read_obj(path):
+    verts, faces = [], []
+    with open(path) as fh:
+        for line in fh:
+            tok = line.split()
+            if not tok:
+                continue
+            if tok[0] == "v":
+                verts.append([float(t) for t in tok[1:4]])
+            elif tok[0] == "f":
+                faces.append([int(t.split("/")[0]) - 1 for t in tok[1:4]])
+    return np.array(verts, np.float32), np.array(faces, np.int32)
+
+
+def icosphere(level, center, radius):
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
+         (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    v = [np.array(p, float) / np.linalg.norm(p) for p in v]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9),
+         (5, 11, 4), (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2),
+         (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        mid = {}
+
+        def m(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                p = v[a] + v[b]
+                v.append(p / np.linalg.norm(p))
+                mid[key] = len(v) - 1
+            return mid[key]
+        nf = []
+        for a, b, c in f:
+            ab, bc, ca = m(a, b), m(b, c), m(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        f = nf
+    verts = (np.array(v) * radius + np.array(center)).astype(np.float32)
+    return verts, np.array(f, np.int32)
+
+
+def write_obj(path, verts, faces):
+    with open(path, "w") as fh:
+        for p in verts:
+            fh.write("v %.9g %.9g %.9g\n" % tuple(float(c) for c in p))
+        for a, b, c in faces:
+            fh.write(f"f {a + 1} {b + 1} {c + 1}\n")
+
+
+def oracle_inside(path, dims, spacing):
+    mesh = O.Mesh(path)
+    ok, internal = mesh.insert(dims, spacing)
+    assert ok
+    return internal.astype(np.uint8)
+
+
+def gpu_inside(verts, faces, dims, spacing):
+    with capi.Context(capi.make_config(*dims, lattice=27,
+                                       mode=capi.MODE_REFERENCE)) as c:
+        return capi.voxelize_mesh(c, verts, faces, spacing)
+
+
+@pytest.mark.parametrize("name", ["cube.obj", "compat_cube.obj",
+                                  "cube_angled.obj"])
+@pytest.mark.parametrize("dims,spacing", [((32, 32, 32), 0.05),
+                                          ((40, 36, 30), 0.03),
+                                          ((10, 10, 10), 0.1)])
+def test_voxelize_reference_meshes(name, dims, spacing):
+    path = os.path.join(DATA, name)
+    verts, faces = read_obj(path)
+    b = np.array([verts.min(0), verts.max(0)])
+    if (b[0] < 0).any() or (b[1] > np.array(dims) * spacing).any():
+        pytest.skip("mesh not inside the box (insertModel ignores it)")
+    say(f"case: voxelise {name} {dims} spacing {spacing}")
+    g = gpu_inside(verts, faces, dims, spacing)
+    o = oracle_inside(path, dims, spacing)
+    say(f"  inside {int(g.sum())} (oracle {int(o.sum())})")
+    assert np.array_equal(g, o)
+
+
+def test_voxelize_sphere_mesh(tmp_path):
+    verts, faces = icosphere(3, (0.6, 0.6, 0.6), 0.45)
+    path = str(tmp_path / "sphere.obj")
+    write_obj(path, verts, faces)
+    verts, faces = read_obj(path)  # the floats the OBJ holds
+    dims, spacing = (48, 48, 48), 0.025
+    say(f"case: voxelise sphere ({len(faces)} faces) {dims}")
+    t0 = time.perf_counter()
+    g = gpu_inside(verts, faces, dims, spacing)
+    t1 = time.perf_counter()
+    o = oracle_inside(path, dims, spacing)
+    t2 = time.perf_counter()
+    say(f"  inside {int(g.sum())}  gpu {t1 - t0:.3f} s (incl. setup)  "
+        f"oracle {t2 - t1:.3f} s")
+    assert np.array_equal(g, o)
+    # the sphere's volume within the lattice resolution
+    expect = 4.0 / 3.0 * np.pi * 0.45 ** 3 / spacing ** 3
+    assert abs(g.sum() - expect) < 0.05 * expect

@@ -187,7 +187,22 @@ def reference_mode_section(steps):
                      "us_per_step": round(us, 2),
                      "reflection_records": t["reflection_records"],
                      "hbm_frac_216B": round(n * 216 / (us * 1e-6) / 8e12, 4)}
+        if name == "fused":
+            out["setup_s_gpu"] = round(t["setup_seconds"], 4)
     out["speedup"] = round(out["fused"]["mlups"] / out["staged"]["mlups"], 3)
+    # the same voxelisation + reflection tables with the host builders
+    # (one process, no GPU work), for the setup comparison
+    dump = os.path.join(ROOT, "latticeboltzmanncuda_amd", "bin",
+                        "GeometryDump")
+    tmp = os.path.join("/tmp", f"lbm_geometry_{os.getpid()}.bin")
+    t0 = time.perf_counter()
+    r = subprocess.run([dump, mesh, "128", "128", "128", "0.0125", tmp],
+                       capture_output=True, text=True, timeout=600)
+    out["setup_s_host"] = round(time.perf_counter() - t0, 4)
+    if os.path.exists(tmp):
+        os.remove(tmp)
+    if r.returncode != 0:
+        raise RuntimeError(f"host geometry builder failed: {r.stderr}")
     return out
 
 

@@ -345,6 +345,18 @@ int lbm_voxelize_mesh(lbm_ctx* ctx, const float* vertices, int32_t nv,
                       const int32_t* faces, int32_t nf, float spacing,
                       uint8_t* inside, int64_t* count);
 
+/* The reflection tables of preProcessModel (Lattice.cpp:323-426, 450-477)
+ * built on the GPU from the mesh and the context's current flags
+ * (LBM_FLAG_INTERNAL as uploaded): one record per internal node with at
+ * least one eligible link, in node order -- node[k], 14 packed direction
+ * words dirs[14k..], 81 weights weights[81k..] -- the values the host
+ * builder gives.  count receives the number of records; it is an error
+ * (with count set) when that exceeds max_records. */
+int lbm_build_reflections(lbm_ctx* ctx, const float* vertices, int32_t nv,
+                          const int32_t* faces, int32_t nf, float spacing,
+                          int32_t max_records, int32_t* node, uint32_t* dirs,
+                          float* weights, int32_t* count);
+
 /* ------------------------------------------------------------ measurement */
 /* No counterpart in the reference: copies a `bytes`-sized device buffer
  * into another with 16-byte accesses, iters times, and reports the

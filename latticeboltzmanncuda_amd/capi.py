@@ -184,6 +184,8 @@ _SIGNATURES = {
     "lbm_batch_residuals": (C.c_int, [P, P, i32]),
     "lbm_serve_run": (C.c_int, [C.POINTER(ServeParams), C.POINTER(Request),
                                 i32, C.POINTER(ServeStats)]),
+    "lbm_build_reflections": (C.c_int, [CTX, P, i32, P, i32, C.c_float, i32,
+                                         P, P, P, P]),
     "lbm_voxelize_mesh": (C.c_int, [CTX, P, i32, P, i32, C.c_float, P, P]),
     "lbm_measure_copy_bandwidth": (C.c_int, [i64, C.c_int,
                                              C.POINTER(C.c_double)]),

@@ -264,19 +264,36 @@ std::array<std::pair<float, int>, 27> Lattice::distributeVector(Point_3 vector) 
   return lbm::distribute_vector(vector);
 }
 
-// Lattice.cpp:323-426.  Records are rebuilt from scratch on every call.
+// Lattice.cpp:323-426.  Records are rebuilt from scratch on every call;
+// the tables are built on the GPU (lbm_build_reflections) with the values
+// of the host builder lbm::build_reflections, which remains for checks.
 void Lattice::preProcessModel() {
   if (!m_simModel) return;
   std::vector<uint8_t> flags = downloadFlags();
-  lbm::build_reflections(*m_simModel, m_xResolution, m_yResolution,
-                         m_zResolution, m_latticeSpacing, flags,
-                         m_reflections, m_reflectionNodes);
-  const std::size_t nrec = m_reflections.size();
-  std::vector<uint32_t> dirs(14 * nrec);
-  std::vector<float> weights(81 * nrec);
+  int32_t ninternal = 0;
+  for (uint8_t f : flags) ninternal += (f & LBM_FLAG_INTERNAL) ? 1 : 0;
+  const TriangleMesh& mesh = m_simModel->getMesh();
+  std::vector<int32_t> nodes((std::size_t)ninternal);
+  std::vector<uint32_t> dirs(14 * (std::size_t)ninternal);
+  std::vector<float> weights(81 * (std::size_t)ninternal);
+  int32_t count = 0;
+  lbm::check(lbm_build_reflections(m_ctx, mesh.vertices.data(), (int32_t)mesh.number_of_vertices(),
+                                   mesh.faces.data(), (int32_t)mesh.number_of_faces(),
+                                   m_latticeSpacing, ninternal, nodes.data(), dirs.data(),
+                                   weights.data(), &count),
+             "lbm_build_reflections");
+  const std::size_t nrec = (std::size_t)count;
+  nodes.resize(nrec);
+  dirs.resize(14 * nrec);
+  weights.resize(81 * nrec);
+  m_reflections.clear();
+  m_reflectionNodes = nodes;
   for (std::size_t r = 0; r < nrec; ++r) {
-    std::memcpy(&dirs[14 * r], m_reflections[r].reflection_directions, 14 * sizeof(uint32_t));
-    std::memcpy(&weights[81 * r], m_reflections[r].reflection_weight, 81 * sizeof(float));
+    m_reflections.emplace_back();
+    ReflectionValues& rec = m_reflections.back();
+    std::memcpy(rec.reflection_directions, &dirs[14 * r], 14 * sizeof(uint32_t));
+    std::memcpy(rec.reflection_weight, &weights[81 * r], 81 * sizeof(float));
+    flags[(std::size_t)nodes[r]] |= LBM_FLAG_REFLECTED;
   }
   lbm::check(lbm_upload_flags(m_ctx, flags.data()), "lbm_upload_flags");
   lbm::check(lbm_upload_reflections(m_ctx, (int32_t)nrec, m_reflectionNodes.data(),

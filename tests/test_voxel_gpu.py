@@ -121,3 +121,50 @@ def test_voxelize_sphere_mesh(tmp_path):
     # the sphere's volume within the lattice resolution
     expect = 4.0 / 3.0 * np.pi * 0.45 ** 3 / spacing ** 3
     assert abs(g.sum() - expect) < 0.05 * expect
+
+
+BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                   "latticeboltzmanncuda_amd", "bin")
+
+
+def dump(tmp_path, mesh_path, dims, spacing, gpu):
+    import subprocess
+    out = tmp_path / ("gpu.bin" if gpu else "host.bin")
+    cmd = [os.path.join(BIN, "GeometryDump"), mesh_path, *map(str, dims),
+           repr(spacing), str(out)] + (["--gpu"] if gpu else [])
+    r = subprocess.run(cmd, timeout=300, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return out.read_bytes()
+
+
+@pytest.mark.parametrize("mesh,dims,spacing", [
+    ("cube_angled.obj", (3, 3, 3), 0.5),
+    ("compat_cube.obj", (32, 32, 32), 0.05),
+    ("cube_angled.obj", (32, 32, 32), 0.05),
+    ("cube_angled.obj", (24, 20, 28), 0.045),
+    ("sphere", (40, 40, 40), 0.03),
+])
+def test_gpu_reflection_tables_equal_host_builder(tmp_path, mesh, dims,
+                                                  spacing):
+    # the GPU voxeliser + reflection-table builder against the host
+    # builders (which match the oracle): the same bytes -- flags, record
+    # nodes, direction words and weights
+    if mesh == "sphere":
+        verts, faces = icosphere(2, (0.6, 0.6, 0.6), 0.4)
+        path = str(tmp_path / "sphere.obj")
+        write_obj(path, verts, faces)
+    else:
+        path = os.path.join(DATA, mesh)
+    say(f"case: reflection tables {mesh} {dims} spacing {spacing}")
+    t0 = time.perf_counter()
+    host = dump(tmp_path, path, dims, spacing, False)
+    t1 = time.perf_counter()
+    gpu = dump(tmp_path, path, dims, spacing, True)
+    t2 = time.perf_counter()
+    n = int(np.prod(dims))
+    nrec = int(np.frombuffer(host[n:n + 4], np.int32)[0])
+    say(f"  records {nrec}  host {t1 - t0:.2f} s  gpu {t2 - t1:.2f} s "
+        "(process wall time)")
+    assert nrec > 0
+    assert len(host) == len(gpu)
+    assert host == gpu

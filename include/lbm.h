@@ -357,6 +357,19 @@ int lbm_build_reflections(lbm_ctx* ctx, const float* vertices, int32_t nv,
                           int32_t max_records, int32_t* node, uint32_t* dirs,
                           float* weights, int32_t* count);
 
+/* ------------------------------------------------------------ snapshots */
+/* No counterpart in the reference (its state leaves the GPU only as the
+ * 144-byte record stream of retrieve_data / load_data, Lattice.cpp:73-113):
+ * writes everything needed to resume -- configuration, step count, the
+ * current populations bit for bit, the standard mode's link mask and class
+ * map or the reference mode's flags, reflection records, flow injection
+ * and force register -- to one binary file. */
+int lbm_snapshot_save(lbm_ctx* ctx, const char* path);
+/* Restores a snapshot into a context created with the same extents,
+ * lattice, storage and mode; tau, body force, moving wall, periodicity,
+ * flow and force come from the snapshot.  Nothing changes on error. */
+int lbm_snapshot_load(lbm_ctx* ctx, const char* path);
+
 /* ------------------------------------------------------------ measurement */
 /* No counterpart in the reference: copies a `bytes`-sized device buffer
  * into another with 16-byte accesses, iters times, and reports the

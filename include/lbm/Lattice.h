@@ -115,6 +115,17 @@ class Lattice {
   // populations are bit-identical to the four calls.
   void simulateStep();
 
+  // Files (no counterpart in the reference).  writeRecords / readRecords:
+  // the retrieve_data / load_data record stream (Lattice.cpp:73-113) as a
+  // file -- "LBMPTS1\0", the three extents as int32, then one 144-byte
+  // LatticePoint per node in node order (reflection pointers not stored).
+  // saveSnapshot / loadSnapshot: the whole resumable state
+  // (lbm_snapshot_save / lbm_snapshot_load).
+  void writeRecords(const std::string& path);
+  void readRecords(const std::string& path);
+  void saveSnapshot(const std::string& path);
+  void loadSnapshot(const std::string& path);
+
   void insertModel(std::string filename);
   std::array<std::pair<float, int>, 27> distributeVector(Point_3 vector);
   void preProcessModel();

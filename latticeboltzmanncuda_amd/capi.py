@@ -186,6 +186,8 @@ _SIGNATURES = {
                                 i32, C.POINTER(ServeStats)]),
     "lbm_build_reflections": (C.c_int, [CTX, P, i32, P, i32, C.c_float, i32,
                                          P, P, P, P]),
+    "lbm_snapshot_save": (C.c_int, [CTX, C.c_char_p]),
+    "lbm_snapshot_load": (C.c_int, [CTX, C.c_char_p]),
     "lbm_voxelize_mesh": (C.c_int, [CTX, P, i32, P, i32, C.c_float, P, P]),
     "lbm_measure_copy_bandwidth": (C.c_int, [i64, C.c_int,
                                              C.POINTER(C.c_double)]),
@@ -445,6 +447,17 @@ class Context:
         check(self.lib.lbm_test_coordinates(self.h, C.byref(ok)),
               "lbm_test_coordinates")
         return bool(ok.value)
+
+    def save(self, path: str):
+        check(self.lib.lbm_snapshot_save(self.h, str(path).encode()),
+              "lbm_snapshot_save")
+
+    def load(self, path: str):
+        check(self.lib.lbm_snapshot_load(self.h, str(path).encode()),
+              "lbm_snapshot_load")
+        cfg = Config()
+        check(self.lib.lbm_get_config(self.h, C.byref(cfg)), "lbm_get_config")
+        self.cfg = cfg
 
     def device_bytes(self) -> int:
         return int(self.lib.lbm_device_bytes(self.h))

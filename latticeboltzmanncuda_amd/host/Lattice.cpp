@@ -5,6 +5,10 @@
 // bytes and reflection records.
 #include "Lattice.h"
 
+#include <cstdio>
+#include <cstring>
+#include <memory>
+
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -118,6 +122,49 @@ void Lattice::simulateFlow() {
 }
 
 void Lattice::simulateLattice() { lbm::check(lbm_step(m_ctx, 1), "lbm_step"); }
+
+void Lattice::writeRecords(const std::string& path) {
+  const int64_t n = (int64_t)m_xResolution * m_yResolution * m_zResolution;
+  std::unique_ptr<LatticePoint[]> pts(retrieve_data());
+  std::FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) throw lbm::Error(LBM_ERR_INVALID, "cannot open " + path);
+  const char magic[8] = {'L', 'B', 'M', 'P', 'T', 'S', '1', '\0'};
+  const int32_t dims[3] = {m_xResolution, m_yResolution, m_zResolution};
+  bool ok = std::fwrite(magic, 1, 8, f) == 8 && std::fwrite(dims, 4, 3, f) == 3;
+  for (int64_t k = 0; ok && k < n; ++k) {
+    LatticePoint p = pts[k];
+    p.reflections = nullptr;  // a host address: meaningless in a file
+    ok = std::fwrite(&p, sizeof(LatticePoint), 1, f) == 1;
+  }
+  if (std::fclose(f) != 0 || !ok) throw lbm::Error(LBM_ERR_INVALID, "write failed: " + path);
+}
+
+void Lattice::readRecords(const std::string& path) {
+  const int64_t n = (int64_t)m_xResolution * m_yResolution * m_zResolution;
+  std::FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) throw lbm::Error(LBM_ERR_INVALID, "cannot open " + path);
+  char magic[8];
+  int32_t dims[3];
+  std::unique_ptr<LatticePoint[]> pts(new LatticePoint[(std::size_t)n]());
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "LBMPTS1", 8) == 0 &&
+            std::fread(dims, 4, 3, f) == 3 && dims[0] == m_xResolution &&
+            dims[1] == m_yResolution && dims[2] == m_zResolution &&
+            std::fread(pts.get(), sizeof(LatticePoint), (std::size_t)n, f) == (std::size_t)n;
+  std::fclose(f);
+  if (!ok) throw lbm::Error(LBM_ERR_INVALID, "not a record file of this lattice: " + path);
+  load_data(pts.get());
+}
+
+void Lattice::saveSnapshot(const std::string& path) {
+  lbm::check(lbm_snapshot_save(m_ctx, path.c_str()), "lbm_snapshot_save");
+}
+
+void Lattice::loadSnapshot(const std::string& path) {
+  lbm::check(lbm_snapshot_load(m_ctx, path.c_str()), "lbm_snapshot_load");
+  lbm_config cfg{};
+  lbm::check(lbm_get_config(m_ctx, &cfg), "lbm_get_config");
+  m_fluid.m_characteristicTimescale = cfg.tau;
+}
 
 void Lattice::simulateStep() {
   lbm::check(lbm_step_reference(m_ctx, 1, m_reflectionData != nullptr ? 1 : 0),

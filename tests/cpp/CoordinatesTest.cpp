@@ -1,5 +1,6 @@
 // Re-expression of the reference's test/CoordinatesTest.cpp:14-229 through
 // the Lattice facade (GPU).
+#include <cstdio>
 #include <memory>
 
 #include "Constants.h"
@@ -101,6 +102,42 @@ TEST(CoordinatesTest, TestReflection) {
   EXPECT_EQ(lat.getReflectionDirection(&centre, 4), 4);
   EXPECT_EQ(lat.getReflectionDirection(&centre, 5), 26);
   EXPECT_TRUE(centre.isReflected && centre.isInternal);
+}
+
+// Not in the reference: the record-stream file and the snapshot file
+// round-trip through a second lattice.
+TEST(CoordinatesTest, TestRecordAndSnapshotFiles) {
+  dim3 threads(4, 3, 5), blocks(2, 2, 1);
+  const int n = 8 * 6 * 5;
+  Lattice a(8, 6, 5, blocks, threads, FluidData(1.0, 0.9), 0.1f);
+  std::unique_ptr<LatticePoint[]> pts(new LatticePoint[n]());
+  for (int k = 0; k < n; ++k)
+    for (int i = 0; i < 27; ++i) pts[k].particle_distribution[i] = 0.001f * (1 + (k * 27 + i) % 97);
+  a.load_data(pts.get());
+  a.writeRecords("/tmp/lbm_test_records.bin");
+  a.simulateLattice();
+  a.saveSnapshot("/tmp/lbm_test_snapshot.bin");
+  a.simulateLattice();
+  std::unique_ptr<LatticePoint[]> want(a.retrieve_data());
+
+  Lattice b(8, 6, 5, blocks, threads, FluidData(1.0, 0.9), 0.1f);
+  b.readRecords("/tmp/lbm_test_records.bin");
+  std::unique_ptr<LatticePoint[]> first(b.retrieve_data());
+  int bad = 0;
+  for (int k = 0; k < n; ++k)
+    for (int i = 0; i < 27; ++i)
+      bad += first[k].particle_distribution[i] != pts[k].particle_distribution[i];
+  EXPECT_EQ(bad, 0);
+  b.loadSnapshot("/tmp/lbm_test_snapshot.bin");
+  b.simulateLattice();
+  std::unique_ptr<LatticePoint[]> got(b.retrieve_data());
+  bad = 0;
+  for (int k = 0; k < n; ++k)
+    for (int i = 0; i < 27; ++i)
+      bad += got[k].particle_distribution[i] != want[k].particle_distribution[i];
+  EXPECT_EQ(bad, 0);
+  std::remove("/tmp/lbm_test_records.bin");
+  std::remove("/tmp/lbm_test_snapshot.bin");
 }
 
 int main(int argc, char** argv) { return RUN_ALL_TESTS(); }

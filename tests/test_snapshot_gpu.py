@@ -1,0 +1,107 @@
+"""Snapshots (lbm_snapshot_save / lbm_snapshot_load): a run interrupted
+by save + load into a fresh context continues bit for bit like the
+uninterrupted run, in standard mode (FP32, FP16) and in reference mode
+(flags, reflection records, flow injection, force register)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from latticeboltzmanncuda_amd import capi, workloads
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                    "dataFiles")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("q,dims,storage", [
+    (19, (40, 24, 20), capi.STORE_F32),
+    (9, (96, 64, 1), capi.STORE_F16),
+    (19, (24, 20, 16), capi.STORE_BF16),
+])
+def test_standard_snapshot_resumes_bit_exact(tmp_path, q, dims, storage):
+    say(f"case: snapshot standard D{3 if q != 9 else 2}Q{q} {dims} storage {storage}")
+    if q == 9:
+        solid = workloads.random_squares(dims[0], dims[1], 0.1,
+                                         np.random.default_rng(3))
+        kw = dict(tau=0.7, lid_face=3, lid_u=(0.08, 0, 0))
+    else:
+        solid, _ = capi.random_spheres(dims, 5, 0.15, 2.0, 4.0, True)
+        kw = dict(tau=0.8, periodic=(1, 0, 0), force=(2e-5, 0, 0))
+    cfg = capi.make_config(*dims, lattice=q, storage=storage, **kw)
+    path = tmp_path / "snap.bin"
+    with capi.Context(cfg) as a:
+        a.upload_solid(solid)
+        a.init_equilibrium()
+        a.step(31)  # odd: the current buffer is the second one
+        a.save(path)
+        a.step(20)
+        want = a.populations()
+    # a context created with other parameters takes the snapshot's
+    other = capi.make_config(*dims, lattice=q, storage=storage, tau=1.3)
+    with capi.Context(other) as b:
+        b.load(path)
+        assert abs(b.cfg.tau - kw["tau"]) < 1e-7
+        b.step(20)
+        got = b.populations()
+    assert np.array_equal(got, want)
+
+
+def test_reference_snapshot_resumes_bit_exact(tmp_path):
+    say("case: snapshot reference mode 24^3 cube_angled, flow, force")
+    dims, spacing, tau = (24, 24, 24), 0.05, 0.6
+    mesh = O.Mesh(os.path.join(DATA, "cube_angled.obj"))
+    ok, internal = mesh.insert(dims, spacing)
+    assert ok
+    flags, recs = mesh.preprocess(dims, spacing, internal * O.FLAG_INTERNAL)
+    n = int(np.prod(dims))
+    flow = np.full(27, 0.001, np.float32)
+    flow[6] = 0.002
+    f = np.tile(flow[:, None], (1, n)).reshape(-1).copy()
+    cfg = capi.make_config(*dims, lattice=27, mode=capi.MODE_REFERENCE,
+                           tau=tau)
+    path = tmp_path / "ref.bin"
+    with capi.Context(cfg) as a:
+        a.set_populations(f)
+        a.upload_flags(flags)
+        a.upload_reflections(*recs)
+        a.set_flow(capi.FLOW_PLANE, [0, 0], flow)
+        a.set_force([0, 0, 0])
+        a.step_reference(15, with_force=True)
+        a.save(path)
+        a.step_reference(10, with_force=True)
+        want, want_F = a.populations(), a.force()
+    with capi.Context(cfg) as b:
+        b.load(path)
+        assert np.array_equal(b.flags(), flags)
+        b.step_reference(10, with_force=True)
+        got, got_F = b.populations(), b.force()
+    assert np.array_equal(got, want)
+    assert np.array_equal(got_F, want_F)
+
+
+def test_snapshot_rejects_other_lattices(tmp_path):
+    say("case: snapshot into a different lattice")
+    path = tmp_path / "s.bin"
+    with capi.Context(capi.make_config(16, 16, 16, lattice=19)) as a:
+        a.init_equilibrium()
+        a.save(path)
+    with capi.Context(capi.make_config(16, 16, 8, lattice=19)) as b:
+        b.init_equilibrium()
+        before = b.populations()
+        with pytest.raises(capi.LbmError):
+            b.load(path)
+        assert np.array_equal(b.populations(), before)
+    with open(path, "r+b") as fh:  # truncated file
+        fh.truncate(200)
+    with capi.Context(capi.make_config(16, 16, 16, lattice=19)) as c:
+        with pytest.raises(capi.LbmError):
+            c.load(path)

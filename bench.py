@@ -263,7 +263,7 @@ def main():
                                          True)
     n = int(np.prod(DIMS))
     fluid = int((solid == 0).sum())
-    # practical HBM ceiling: a plain copy of one population array
+    # practical HBM ceiling: a streaming copy of one population array
     copy_gbs = capi.measure_copy_bandwidth(n * 19 * 4, 20)
     cfg = capi.make_config(*DIMS, lattice=capi.D3Q19,
                            storage=capi.STORE_F32, tau=TAU,

@@ -985,9 +985,9 @@ int lbm_measure_copy_bandwidth(int64_t bytes, int iters, double* gbs) {
   float ms = 0.0f;
   rc = LBM_OK;
   if ((e = hipMemsetAsync(a.p, 0, nbytes, s)) != hipSuccess) rc = hip_fail(e, "hipMemsetAsync");
-  for (int w = 0; w < 3 && rc == LBM_OK; ++w) rc = launch_copy4(a.p, b.p, nbytes, s);
+  for (int w = 0; w < 3 && rc == LBM_OK; ++w) rc = launch_copy_stream(a.p, b.p, nbytes, s);
   if (rc == LBM_OK && (e = hipEventRecord(e0, s)) != hipSuccess) rc = hip_fail(e, "hipEventRecord");
-  for (int k = 0; k < iters && rc == LBM_OK; ++k) rc = launch_copy4(a.p, b.p, nbytes, s);
+  for (int k = 0; k < iters && rc == LBM_OK; ++k) rc = launch_copy_stream(a.p, b.p, nbytes, s);
   if (rc == LBM_OK && (e = hipEventRecord(e1, s)) != hipSuccess) rc = hip_fail(e, "hipEventRecord");
   if (rc == LBM_OK && (e = hipEventSynchronize(e1)) != hipSuccess) rc = hip_fail(e, "hipEventSynchronize");
   if (rc == LBM_OK && (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) rc = hip_fail(e, "hipEventElapsedTime");

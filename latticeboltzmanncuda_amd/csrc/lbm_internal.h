@@ -149,7 +149,7 @@ int launch_points_out(int q, int storage, const Grid& g, const void* f,
                       hipStream_t s);
 int launch_test_coordinates(const Grid& g, uint32_t* counts, int* ok,
                             hipStream_t s);
-int launch_copy4(const void* src, void* dst, uint64_t bytes, hipStream_t s);
+int launch_copy_stream(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 
 }  // namespace lbm
 

@@ -214,16 +214,31 @@ def cpu_baseline(solid: np.ndarray, seconds: float) -> dict:
     q = 19
     mask = O.std_build_mask(q, DIMS, solid, (1, 0, 0))
     p = O.std_params(tau=TAU, force=FORCE, periodic=(1, 0, 0))
-    f = O.std_init_eq(q, DIMS).astype(np.float32)
+    f0 = O.std_init_eq(q, DIMS).astype(np.float32)
     n = int(np.prod(DIMS))
     threads = O.get_threads()
-    f = O.std_run_f32(q, DIMS, f, mask, p, 1)  # warm-up step
-    steps, t0 = 0, time.perf_counter()
     chunk = 8
-    while time.perf_counter() - t0 < seconds:
-        f = O.std_run_f32(q, DIMS, f, mask, p, chunk)
-        steps += chunk
-    dt = time.perf_counter() - t0
+
+    def timed(budget, nthreads):
+        O.set_threads(nthreads)
+        f = O.std_run_f32(q, DIMS, f0, mask, p, 1)  # warm-up step
+        steps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget:
+            f = O.std_run_f32(q, DIMS, f, mask, p, chunk)
+            steps += chunk
+        return steps, time.perf_counter() - t0
+
+    steps, dt = timed(seconds, threads)
+    # one thread, a quarter of the budget (a scalar-port reference point)
+    steps1, dt1 = timed(seconds / 4, 1)
+    O.set_threads(threads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((l.split(":", 1)[1].strip() for l in fh
+                          if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": round(n * steps / dt / 1e6, 2),
         "unit": "MLUPS",
@@ -233,6 +248,10 @@ def cpu_baseline(solid: np.ndarray, seconds: float) -> dict:
                    f"threads) on the same 256x128x128 sphere channel, "
                    f"{steps} steps in {dt:.1f} s; includes a per-call "
                    f"population copy-in/out every {chunk} steps"),
+        "single_thread_value": round(n * steps1 / dt1 / 1e6, 2),
+        "single_thread_sample": f"{steps1} steps in {dt1:.1f} s, 1 thread",
+        "cpu_model": model,
+        "nproc": os.cpu_count(),
     }
 
 

@@ -52,3 +52,5 @@ def test_cpu_baseline_leg_runs(monkeypatch):
     assert out["kind"] == "port" and out["unit"] == "MLUPS"
     assert out["value"] > 0 and out["cores"] >= 1
     assert np.isfinite(out["value"])
+    assert out["single_thread_value"] > 0
+    assert out["nproc"] >= 1

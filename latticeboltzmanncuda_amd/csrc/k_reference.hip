@@ -29,6 +29,7 @@
 #include "lbm_internal.h"
 #include "lbm_lattice.h"
 #include "lbm_layout.h"
+#include "lbm_store.h"
 
 #pragma clang fp contract(off)
 
@@ -69,13 +70,20 @@ __device__ __forceinline__ bool flow_match(const FlowSpec& fl, uint32_t n,
 __device__ __forceinline__ void pull27(const Grid& g, const float* __restrict__ A,
                                        uint32_t n, int x, int y, int z,
                                        float f[27]) {
+  // per axis, indexed by e + 1 for e = -1, 0, +1: whether the source
+  // coordinate x - e lies in the box, and the node offset of x - e (mod 2^32)
+  const uint32_t plane = g.ny * g.nz;
+  const bool bx[3] = {x + 1 < (int)g.nx, true, x > 0};
+  const bool by[3] = {y + 1 < (int)g.ny, true, y > 0};
+  const bool bz[3] = {z + 1 < (int)g.nz, true, z > 0};
+  const uint32_t ox[3] = {plane, 0u, 0u - plane};
+  const uint32_t oy[3] = {g.nz, 0u, 0u - g.nz};
+  const uint32_t oz[3] = {1u, 0u, 0u - 1u};
 #pragma unroll
   for (int j = 0; j < 27; ++j) {
-    const int sx = x - L27::E[j][0], sy = y - L27::E[j][1],
-              sz = z - L27::E[j][2];
-    const bool in = sx >= 0 && sy >= 0 && sz >= 0 && sx < (int)g.nx &&
-                    sy < (int)g.ny && sz < (int)g.nz;
-    const uint32_t src = in ? (uint32_t)sz + g.nz * ((uint32_t)sy + g.ny * (uint32_t)sx) : n;
+    const int a = L27::E[j][0] + 1, b = L27::E[j][1] + 1, c = L27::E[j][2] + 1;
+    const bool in = bx[a] && by[b] && bz[c];
+    const uint32_t src = in ? n + ox[a] + oy[b] + oz[c] : n;
     const float v = A[pidx<27>(src, j)];
     f[j] = in ? v : 0.0f;
   }
@@ -226,6 +234,36 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
+// Called by all kBlock threads of one workgroup: sums the partial forces
+// (3 per entry) in FP64 in a fixed order -- thread t takes entries t,
+// t + kBlock, ...; then a fixed tree -- and adds the total to the FP32
+// force register: one rounding per reflection pass, the same result on
+// every run.
+__device__ void force_sum_block(const float* __restrict__ partials, int nblocks,
+                                float* __restrict__ force) {
+  __shared__ double red[3][kBlock];
+  const int t = threadIdx.x;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int b = t; b < nblocks; b += kBlock) {
+    s0 += (double)partials[3 * b];
+    s1 += (double)partials[3 * b + 1];
+    s2 += (double)partials[3 * b + 2];
+  }
+  red[0][t] = s0;
+  red[1][t] = s1;
+  red[2][t] = s2;
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if (t < h) {
+      red[0][t] += red[0][t + h];
+      red[1][t] += red[1][t + h];
+      red[2][t] += red[2][t + h];
+    }
+    __syncthreads();
+  }
+  if (t < 3) force[t] = (float)((double)force[t] + red[t][0]);
+}
+
 // One thread per node with a nonzero flag byte.  Slot s of a record holds a
 // 5-bit direction d in word s/6, bits 5*(s%6) (Lattice.cpp:450-477); its
 // contribution f_{s/3}(x) * W[s] goes to (x + e_d, d).  The force term uses
@@ -292,35 +330,70 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// e_s SPLIT_s of slot s per axis (0 for s >= 27, outside the reference's
+// tables): the second factor of the force term of SimCalcFuncs.cu:181-183.
+struct SlotTerms {
+  float v[81][3];
+};
+constexpr SlotTerms make_slot_terms() {
+  SlotTerms t{};
+  for (int s = 0; s < 27; ++s)
+    for (int a = 0; a < 3; ++a) t.v[s][a] = (float)L27::E[s][a] * kSplit27[s];
+  return t;
+}
+__constant__ constexpr SlotTerms kSlotTerms = make_slot_terms();
+
 // Gather pass of the fused reference step: one thread per group (t, j).
 // The group's value is the staged pipeline's f_j(t) after reflection:
 // the streamed value A_j(x) of the reflecting node x = t - e_j, then, in
 // ascending slot order, + W[s] * F_k(x) with F_k(x) = A_k(x - e_k) (0
 // outside the box) -- the same additions in the same order as the staged
 // pass.  The force gets val * (e_j SPLIT_j - e_s SPLIT_s) per slot
-// (SimCalcFuncs.cu:181-183), summed per wavefront.
+// (SimCalcFuncs.cu:181-183), summed per wavefront.  Entries are taken four
+// at a time: their loads are issued together, the sums still run in entry
+// order.
 __global__ void __launch_bounds__(kBlock)
     k_ref_reflect_gather(const float* __restrict__ A, uint32_t ngrp,
                          const uint32_t* __restrict__ gsrc,
                          const uint32_t* __restrict__ gbeg, uint32_t nent,
                          const uint32_t* __restrict__ eidx,
                          const float* __restrict__ ew,
-                         const float4* __restrict__ ef, float* __restrict__ gsum,
+                         const uint8_t* __restrict__ es, float* __restrict__ gsum,
                          float* __restrict__ partials) {
+  constexpr int kU = 4;
   const uint32_t gi = blockIdx.x * kBlock + threadIdx.x;
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   if (gi < ngrp) {
-    float acc = A[gsrc[gi]];
+    const uint32_t src = gsrc[gi];
+    float acc = A[src];
     const uint32_t end = gi + 1 < ngrp ? gbeg[gi + 1] : nent;
-    for (uint32_t e = gbeg[gi]; e < end; ++e) {
-      const uint32_t el = eidx[e];
-      const float v = el != ~0u ? A[el] : 0.0f;
-      const float val = v * ew[e];
-      acc = acc + val;
-      const float4 cf = ef[e];
-      fx += val * cf.x;
-      fy += val * cf.y;
-      fz += val * cf.z;
+    // the group's direction j: gsrc is the element index of (x, j)
+    const int j = (int)((src / kTile) % 27u);
+    const float tj0 = (float)L27::E[j][0] * kSplit27[j];
+    const float tj1 = (float)L27::E[j][1] * kSplit27[j];
+    const float tj2 = (float)L27::E[j][2] * kSplit27[j];
+    for (uint32_t e = gbeg[gi]; e < end; e += kU) {
+      uint32_t el[kU], sl[kU];
+      float w[kU], v[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const bool in = e + u < end;
+        el[u] = in ? eidx[e + u] : ~0u;
+        w[u] = in ? ew[e + u] : 0.0f;
+        sl[u] = in ? es[e + u] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) v[u] = el[u] != ~0u ? A[el[u]] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (e + u < end) {
+          const float val = v[u] * w[u];
+          acc = acc + val;
+          fx += val * (tj0 - kSlotTerms.v[sl[u]][0]);
+          fy += val * (tj1 - kSlotTerms.v[sl[u]][1]);
+          fz += val * (tj2 - kSlotTerms.v[sl[u]][2]);
+        }
+      }
     }
     gsum[gi] = acc;
   }
@@ -342,15 +415,11 @@ __global__ void __launch_bounds__(kBlock)
 //      its in-mask; its groups ascend in j from rxbase[n]);
 //   C  collide27; F flow injection.
 // The populations are bit-identical to the staged passes.
-template <bool GEOM>
-__global__ void __launch_bounds__(kBlock)
-    k_ref_step_full(Grid g, const float* __restrict__ A, float* __restrict__ B,
-                    Relax tau, FlowSpec flow, const uint32_t* __restrict__ cls,
-                    const uint32_t* __restrict__ inmask,
-                    const int32_t* __restrict__ rxbase,
-                    const float* __restrict__ gsum) {
-  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
-  if (n >= g.n) return;
+template <bool GEOM, bool NT>
+__device__ __forceinline__ void step_full_node(
+    const Grid& g, const float* __restrict__ A, float* __restrict__ B, const Relax& tau,
+    const FlowSpec& flow, const uint32_t* __restrict__ cls, const uint32_t* __restrict__ inmask,
+    const int32_t* __restrict__ rxbase, const float* __restrict__ gsum, uint32_t n) {
   uint32_t ux, uy, uz;
   decode(g, n, ux, uy, uz);
   const int x = (int)ux, y = (int)uy, z = (int)uz;
@@ -375,39 +444,38 @@ __global__ void __launch_bounds__(kBlock)
     for (int i = 0; i < 27; ++i) f[i] = f[i] + flow.induced[i];
   }
 #pragma unroll
-  for (int i = 0; i < 27; ++i) B[pidx<27>(n, i)] = f[i];
+  for (int i = 0; i < 27; ++i) {
+    if (NT)
+      __builtin_nontemporal_store(f[i], &B[pidx<27>(n, i)]);
+    else
+      B[pidx<27>(n, i)] = f[i];
+  }
 }
 
-// Adds the per-block partial forces of one reflection pass, summed in FP64
-// in a fixed order (thread t takes blocks t, t + 1024, ...; then a fixed
-// tree), to the FP32 force register: one rounding per pass, the same result
-// on every run.
-constexpr int kForceThreads = 1024;
+template <bool GEOM, bool NT>
+__global__ void __launch_bounds__(kBlock)
+    k_ref_step_full(Grid g, const float* __restrict__ A, float* __restrict__ B,
+                    Relax tau, FlowSpec flow, const uint32_t* __restrict__ cls,
+                    const uint32_t* __restrict__ inmask,
+                    const int32_t* __restrict__ rxbase,
+                    const float* __restrict__ gsum, int rev,
+                    const float* __restrict__ partials, int npart,
+                    float* __restrict__ force) {
+  const uint32_t blk = rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
+  const uint32_t n = blk * kBlock + threadIdx.x;
+  if (n < g.n) step_full_node<GEOM, NT>(g, A, B, tau, flow, cls, inmask, rxbase, gsum, n);
+  // the gather pass's force partials are complete when this kernel starts:
+  // the first workgroup dispatched adds them to the force register (its
+  // extra work overlaps the rest of the grid)
+  if (GEOM && force && blockIdx.x == 0) force_sum_block(partials, npart, force);
+}
 
-__global__ void __launch_bounds__(kForceThreads)
+// Adds the per-block partial forces of one reflection pass to the FP32
+// force register; see force_sum_block.
+__global__ void __launch_bounds__(kBlock)
     k_ref_force_sum(const float* __restrict__ partials, int nblocks,
                     float* __restrict__ force) {
-  __shared__ double red[3][kForceThreads];
-  const int t = threadIdx.x;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (int b = t; b < nblocks; b += kForceThreads) {
-    s0 += (double)partials[3 * b];
-    s1 += (double)partials[3 * b + 1];
-    s2 += (double)partials[3 * b + 2];
-  }
-  red[0][t] = s0;
-  red[1][t] = s1;
-  red[2][t] = s2;
-  __syncthreads();
-  for (int h = kForceThreads / 2; h > 0; h >>= 1) {
-    if (t < h) {
-      red[0][t] += red[0][t + h];
-      red[1][t] += red[1][t + h];
-      red[2][t] += red[2][t + h];
-    }
-    __syncthreads();
-  }
-  if (t < 3) force[t] = (float)((double)force[t] + red[t][0]);
+  force_sum_block(partials, nblocks, force);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -481,7 +549,7 @@ int launch_ref_reflect(const Grid& g, float* f, int32_t nspecial,
                      force ? force_partials : nullptr);
   LBM_HIP(hipGetLastError());
   if (force) {
-    hipLaunchKernelGGL(k_ref_force_sum, dim3(1), dim3(kForceThreads), 0, s,
+    hipLaunchKernelGGL(k_ref_force_sum, dim3(1), dim3(kBlock), 0, s,
                        force_partials, nblocks, force);
     LBM_HIP(hipGetLastError());
   }
@@ -497,26 +565,36 @@ int launch_ref_step_full(const Grid& g, const float* A, float* B, float tau,
                          const uint32_t* inmask, const int32_t* rxbase,
                          int64_t ngrp, const uint32_t* gsrc, const uint32_t* gbeg,
                          float* gsum, int64_t nent, const uint32_t* eidx,
-                         const float* ew, const float4* ef, float* partials,
-                         float* force, hipStream_t s) {
+                         const float* ew, const uint8_t* es, float* partials,
+                         float* force, int rev, hipStream_t s) {
   const unsigned nb = blocks_for(g.n);
+  // the force partials of the gather pass, summed by the fused pass
+  const bool with_force = cls && ngrp > 0 && force && partials;
   if (cls && ngrp > 0) {
     hipLaunchKernelGGL(k_ref_reflect_gather, dim3(blocks_for((uint32_t)ngrp)), dim3(kBlock), 0, s,
-                       A, (uint32_t)ngrp, gsrc, gbeg, (uint32_t)nent, eidx, ew, ef, gsum,
-                       force ? partials : nullptr);
+                       A, (uint32_t)ngrp, gsrc, gbeg, (uint32_t)nent, eidx, ew, es, gsum,
+                       with_force ? partials : nullptr);
     LBM_HIP(hipGetLastError());
-    if (force && partials) {
-      hipLaunchKernelGGL(k_ref_force_sum, dim3(1), dim3(kForceThreads), 0, s, partials,
-                         ref_gather_partials(ngrp), force);
-      LBM_HIP(hipGetLastError());
-    }
   }
-  if (cls)
-    hipLaunchKernelGGL(k_ref_step_full<true>, dim3(nb), dim3(kBlock), 0, s, g, A, B,
-                       make_relax(tau), flow, cls, inmask, rxbase, gsum);
+  const int npart = with_force ? ref_gather_partials(ngrp) : 0;
+  float* fsum = with_force ? force : nullptr;
+  // non-temporal stores and alternating tile order once both population
+  // copies exceed the Infinity Cache (as for the standard step)
+  const bool nt = streaming_stores(2ull * 27 * sizeof(float) * g.n);
+  const int r = nt ? rev : 0;
+  const Relax rx = make_relax(tau);
+  if (cls && nt)
+    hipLaunchKernelGGL((k_ref_step_full<true, true>), dim3(nb), dim3(kBlock), 0, s, g, A, B, rx,
+                       flow, cls, inmask, rxbase, gsum, r, partials, npart, fsum);
+  else if (cls)
+    hipLaunchKernelGGL((k_ref_step_full<true, false>), dim3(nb), dim3(kBlock), 0, s, g, A, B, rx,
+                       flow, cls, inmask, rxbase, gsum, r, partials, npart, fsum);
+  else if (nt)
+    hipLaunchKernelGGL((k_ref_step_full<false, true>), dim3(nb), dim3(kBlock), 0, s, g, A, B, rx,
+                       flow, nullptr, nullptr, nullptr, nullptr, r, nullptr, 0, nullptr);
   else
-    hipLaunchKernelGGL(k_ref_step_full<false>, dim3(nb), dim3(kBlock), 0, s, g, A, B,
-                       make_relax(tau), flow, nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((k_ref_step_full<false, false>), dim3(nb), dim3(kBlock), 0, s, g, A, B, rx,
+                       flow, nullptr, nullptr, nullptr, nullptr, r, nullptr, 0, nullptr);
   LBM_HIP(hipGetLastError());
   return LBM_OK;
 }

@@ -239,7 +239,7 @@ int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
   dev_free(c, (void**)&c->ref_gsum, sizeof(float) * ng);
   dev_free(c, (void**)&c->ref_eidx, sizeof(uint32_t) * ne);
   dev_free(c, (void**)&c->ref_ew, sizeof(float) * ne);
-  dev_free(c, (void**)&c->ref_ef, sizeof(float4) * ne);
+  dev_free(c, (void**)&c->ref_es, sizeof(uint8_t) * ne);
   dev_free(c, (void**)&c->ref_partials, sizeof(float) * 3 * (size_t)std::max(c->ref_partials_cap, 1));
   c->ref_ngrp = c->ref_nent = 0;
   c->ref_partials_cap = 0;
@@ -278,14 +278,10 @@ int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
   std::stable_sort(links.begin(), links.end(), [](const Link& a, const Link& b) {
     return a.t != b.t ? a.t < b.t : (a.d != b.d ? a.d < b.d : a.s < b.s);
   });
-  // SimCalcFuncs.cu:181-183: the force term of slot s sent in direction d
-  // is val * (e_d SPLIT_d - e_s SPLIT_s), the second factor taken as 0 for
-  // s >= 27 (outside the reference's tables)
-  auto esplit = [](int i, int a) { return (float)Lat<27>::E[i][a] * kSplit27[i]; };
   std::vector<uint32_t> inmask((size_t)N, 0u), gsrc, gbeg, eidx;
   std::vector<int32_t> base((size_t)N, -1);
   std::vector<float> ew;
-  std::vector<float4> ef;
+  std::vector<uint8_t> es;
   for (size_t i = 0; i < links.size();) {
     const uint32_t t = links[i].t;
     base[t] = (int32_t)gsrc.size();
@@ -305,12 +301,7 @@ int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
         const bool in = sx >= 0 && sy >= 0 && sz >= 0 && sx < g.nx && sy < g.ny && sz < g.nz;
         eidx.push_back(in ? pidx<27>((uint32_t)at(sx, sy, sz), k) : ~0u);
         ew.push_back(c->host_rec_w[81 * (size_t)links[i].r + (size_t)s]);
-        float4 fc;
-        fc.x = esplit(d, 0) - (s < 27 ? esplit(s, 0) : 0.0f);
-        fc.y = esplit(d, 1) - (s < 27 ? esplit(s, 1) : 0.0f);
-        fc.z = esplit(d, 2) - (s < 27 ? esplit(s, 2) : 0.0f);
-        fc.w = 0.0f;
-        ef.push_back(fc);
+        es.push_back((uint8_t)s);
       }
     }
   }
@@ -328,7 +319,7 @@ int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
       (rc = dev_alloc(c, (void**)&c->ref_gsum, sizeof(float) * ng2, "hipMalloc(ref group sums)")) ||
       (rc = dev_alloc(c, (void**)&c->ref_eidx, sizeof(uint32_t) * ne2, "hipMalloc(ref entries)")) ||
       (rc = dev_alloc(c, (void**)&c->ref_ew, sizeof(float) * ne2, "hipMalloc(ref weights)")) ||
-      (rc = dev_alloc(c, (void**)&c->ref_ef, sizeof(float4) * ne2, "hipMalloc(ref force terms)")) ||
+      (rc = dev_alloc(c, (void**)&c->ref_es, sizeof(uint8_t) * ne2, "hipMalloc(ref slots)")) ||
       (rc = dev_alloc(c, (void**)&c->ref_partials, sizeof(float) * 3 * (size_t)std::max(c->ref_partials_cap, 1),
                       "hipMalloc(ref force partials)")))
     return rc;
@@ -341,7 +332,7 @@ int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
        (rc = h2d(c, c->ref_gbeg, gbeg.data(), sizeof(uint32_t) * gbeg.size())) ||
        (rc = h2d(c, c->ref_eidx, eidx.data(), sizeof(uint32_t) * eidx.size())) ||
        (rc = h2d(c, c->ref_ew, ew.data(), sizeof(float) * ew.size())) ||
-       (rc = h2d(c, c->ref_ef, ef.data(), sizeof(float4) * ef.size()))))
+       (rc = h2d(c, c->ref_es, es.data(), sizeof(uint8_t) * es.size()))))
     return rc;
   return LBM_OK;
 }
@@ -440,8 +431,8 @@ int step_reference_full(lbm_ctx* c, int n, int with_force) {
                                    c->cfg.tau, c->flow, geom ? c->ref_cls : nullptr,
                                    c->ref_inmask, c->ref_rxbase, c->ref_ngrp, c->ref_gsrc,
                                    c->ref_gbeg, c->ref_gsum, c->ref_nent, c->ref_eidx, c->ref_ew,
-                                   c->ref_ef, geom && with_force ? c->ref_partials : nullptr,
-                                   geom && with_force ? c->force : nullptr, c->stream)))
+                                   c->ref_es, geom && with_force ? c->ref_partials : nullptr,
+                                   geom && with_force ? c->force : nullptr, c->cur, c->stream)))
       return rc;
     c->cur ^= 1;
   }
@@ -574,7 +565,7 @@ int lbm_destroy(lbm_ctx* c) {
                   (void*)c->force_partials, (void*)c->red, (void*)c->red_out,
                   (void*)c->ref_cls, (void*)c->ref_inmask, (void*)c->ref_rxbase,
                   (void*)c->ref_gsrc, (void*)c->ref_gbeg, (void*)c->ref_gsum,
-                  (void*)c->ref_eidx, (void*)c->ref_ew, (void*)c->ref_ef,
+                  (void*)c->ref_eidx, (void*)c->ref_ew, (void*)c->ref_es,
                   (void*)c->ref_partials})
     if (p) (void)hipFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -785,7 +776,10 @@ int lbm_upload_reflections(lbm_ctx* c, int32_t count, const int32_t* node,
   c->host_rec_node.assign(node, node + count);
   c->host_rec_dirs.assign(dirs, dirs + 14 * (size_t)count);
   c->host_rec_w.assign(weights, weights + 81 * (size_t)count);
-  return LBM_OK;
+  // the reflection records usually complete the geometry: build the fused
+  // step's receiver tables now rather than in the first step (a later flag
+  // change marks them for rebuilding again)
+  return rebuild_special(c);
 }
 
 int lbm_set_flow(lbm_ctx* c, int kind, const int32_t* params,

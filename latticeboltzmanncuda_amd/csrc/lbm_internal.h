@@ -132,8 +132,8 @@ int launch_ref_step_full(const Grid& g, const float* A, float* B, float tau,
                          const uint32_t* inmask, const int32_t* rxbase,
                          int64_t ngrp, const uint32_t* gsrc, const uint32_t* gbeg,
                          float* gsum, int64_t nent, const uint32_t* eidx,
-                         const float* ew, const float4* ef, float* partials,
-                         float* force, hipStream_t s);
+                         const float* ew, const uint8_t* es, float* partials,
+                         float* force, int rev, hipStream_t s);
 int ref_gather_partials(int64_t ngrp);
 int launch_ref_flow(const Grid& g, float* f, const FlowSpec& flow,
                     hipStream_t s);
@@ -207,7 +207,7 @@ struct lbm_ctx {
   int64_t ref_nent = 0;            // entries (one per reflected slot)
   uint32_t* ref_eidx = nullptr;    // per entry: element index of F_k(x) = A_k(x - e_k), ~0u outside
   float* ref_ew = nullptr;         // per entry: slot weight W[s]
-  float4* ref_ef = nullptr;        // per entry: force coefficients (x, y, z, unused)
+  uint8_t* ref_es = nullptr;       // per entry: slot s (its force term comes from a table)
   float* ref_partials = nullptr;   // force partials of the gather pass, 3 per wavefront
   int ref_partials_cap = 0;
   double* red = nullptr;

@@ -49,6 +49,7 @@ void fill_coef(const StdParams& p, StepCoef<C>& k) {
     k.force[a] = (C)p.force[a];
     k.half_force[a] = (C)0.5 * (C)p.force[a];
   }
+  k.forced = p.force[0] != 0.0f || p.force[1] != 0.0f || p.force[2] != 0.0f;
   for (int i = 0; i < L::Q; ++i) {
     const int* e = L::E[i];
     const C w = (C)L::W[i];

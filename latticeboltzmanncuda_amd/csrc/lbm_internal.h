@@ -55,6 +55,7 @@ struct StepCoef {
   C gw[27];          // (1 - omega / 2) w_i
   C gf3[27];         // 3 (1 - omega / 2) w_i (e_i . F)
   C gf9[27];         // 9 (1 - omega / 2) w_i (e_i . F)
+  int forced;        // F != 0 (else the Guo terms are skipped)
 };
 
 // Parameters of the standard (BGK) step.

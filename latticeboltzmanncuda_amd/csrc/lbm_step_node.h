@@ -105,20 +105,68 @@ __device__ __forceinline__ void std_gather(const Grid& g,
   ld.solid = c == kClassSolid;
   ld.m = c == kClassBoundary ? gload(mask + n) : 0u;
   ld.own = pidx<Q>(n, 0);
-  // solid lanes load nothing and keep the rest state (f = w_i, g = 0):
-  // zero for shifted storage; unshifted storage selects w_i when colliding
+  // solid lanes load nothing: their values are left unspecified (no
+  // instruction spent on them), the collision replaces their outputs with
+  // the rest state
 #pragma unroll
-  for (int i = 0; i < Q; ++i) ld.raw[i] = 0;
+  for (int i = 0; i < Q; ++i) ld.raw[i] = __builtin_nondeterministic_value(ld.raw[i]);
   if (!ld.solid) {
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       const uint32_t src = n + ox[L::E[i][0] + 1] + oy[L::E[i][1] + 1] + oz[L::E[i][2] + 1];
-      const uint32_t from = pidx<Q>(src, i);
+      // pidx(src, i) as one 24-bit multiply-add: lane | i * kTile < Q * kTile
+      const uint32_t from =
+          __umul24(src / kTile, (uint32_t)(Q * kTile)) + ((src % kTile) | ((uint32_t)i * kTile));
       const uint32_t back = ld.own + (uint32_t)L::INV[i] * kTile;
       // bounced link: read the node's own opposite population instead
       const uint32_t sel = 0u - ((ld.m >> i) & 1u);
       ld.raw[i] = gload(elem(A, from ^ ((from ^ back) & sel)));
     }
+  }
+}
+
+// BGK relaxation towards the equilibrium of (rho, u), plus the Guo source
+// term when FORCED, and the stores of the Q outputs (solid nodes: the rest
+// state).
+template <int Q, int S, bool NT, bool FORCED>
+__device__ __forceinline__ void std_relax(const StepCoef<typename Store<S>::C>& k,
+                                          const typename Store<S>::C (&gv)[Q],
+                                          typename Store<S>::C rho, typename Store<S>::C ux,
+                                          typename Store<S>::C uy, typename Store<S>::C uz,
+                                          typename Store<S>::C m3uF, bool solid,
+                                          typename Store<S>::T* out0) {
+#pragma clang fp contract(on)
+  using L = Lat<Q>;
+  using St = Store<S>;
+  using C = typename St::C;
+  constexpr bool kSh = St::kShifted;
+  const C usq = ux * ux + uy * uy + uz * uz;
+  const C base = (C)1 - (C)1.5 * usq;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    // e.u from the nonzero components only (-0 + v == v exactly)
+    C eu = (C)-0.0;
+    if (L::E[i][0]) eu = L::E[i][0] > 0 ? eu + ux : eu - ux;
+    if (L::E[i][1]) eu = L::E[i][1] > 0 ? eu + uy : eu - uy;
+    if (L::E[i][2]) eu = L::E[i][2] > 0 ? eu + uz : eu - uz;
+    const C w = (C)L::W[i];
+    // t = 1 - 1.5 u.u + 3 e.u + 4.5 (e.u)^2, feq = w rho t
+    const C t = fmadd(fmadd((C)4.5, eu, (C)3), eu, base);
+    // feq - f in the stored form: w rho t - w - g (shifted), w rho t - f
+    const C d = (kSh ? fmadd(w * rho, t, -w) : (w * rho) * t) - gv[i];
+    C out = fmadd(k.omega, d, gv[i]);
+    if (FORCED) {
+      // Guo source (1 - omega/2) w (3 (e.F - u.F) + 9 (e.u)(e.F)), added
+      // term by term (each step takes one per-lattice constant)
+      out = out + k.gf3[i];
+      out = fmadd(k.gw[i], m3uF, out);
+      out = fmadd(k.gf9[i], eu, out);
+    }
+    out = solid ? (kSh ? (C)0 : w) : out;
+    if (NT)
+      St::store_nt(out0 + i * kTile, out);
+    else
+      St::store(out0 + i * kTile, out);
   }
 }
 
@@ -144,7 +192,6 @@ __device__ __forceinline__ void std_collide(const StdParams& p,
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     C v = St::cvt(ld.raw[i]);
-    if (!kSh) v = ld.solid ? (C)L::W[i] : v;
     if (LID && i > 0) {  // the rest direction never crosses a face
       const bool wall = lidnode && ((m >> i) & 1u);
       v += wall ? k.lid[i] : (C)0;
@@ -162,32 +209,19 @@ __device__ __forceinline__ void std_collide(const StdParams& p,
     if (L::E[i][2]) jz += L::E[i][2] > 0 ? gv[i] : -gv[i];
   }
   const C ir = recip(rho);
-  const C ux = (jx + k.half_force[0]) * ir;
-  const C uy = (jy + k.half_force[1]) * ir;
-  const C uz = (jz + k.half_force[2]) * ir;
-  const C usq = ux * ux + uy * uy + uz * uz;
-  const C m3uF = (C)-3 * (ux * k.force[0] + uy * k.force[1] + uz * k.force[2]);
-  const C base = (C)1 - (C)1.5 * usq;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    // e.u from the nonzero components only (-0 + v == v exactly)
-    C eu = (C)-0.0;
-    if (L::E[i][0]) eu = L::E[i][0] > 0 ? eu + ux : eu - ux;
-    if (L::E[i][1]) eu = L::E[i][1] > 0 ? eu + uy : eu - uy;
-    if (L::E[i][2]) eu = L::E[i][2] > 0 ? eu + uz : eu - uz;
-    const C w = (C)L::W[i];
-    // t = 1 - 1.5 u.u + 3 e.u + 4.5 (e.u)^2, feq = w rho t
-    const C t = fmadd(fmadd((C)4.5, eu, (C)3), eu, base);
-    // feq - f in the stored form: w rho t - w - g (shifted), w rho t - f
-    const C d = (kSh ? fmadd(w * rho, t, -w) : (w * rho) * t) - gv[i];
-    // Guo source (1 - omega/2) w (3 (e.F - u.F) + 9 (e.u)(e.F))
-    const C src = fmadd(k.gf9[i], eu, fmadd(k.gw[i], m3uF, k.gf3[i]));
-    C out = fmadd(k.omega, d, gv[i]) + src;
-    out = ld.solid ? (kSh ? (C)0 : w) : out;
-    if (NT)
-      St::store_nt(elem(B, ld.own + (uint32_t)i * kTile), out);
-    else
-      St::store(elem(B, ld.own + (uint32_t)i * kTile), out);
+  // the node's Q outputs lie kTile elements apart: one 64-bit address, the
+  // direction offsets become constant offsets of the stores
+  typename St::T* const out0 = B + ld.own;
+  // Guo forcing only where the lattice has a body force (uniform per
+  // lattice); without one, u = j / rho and the source term is zero
+  if (k.forced) {
+    const C ux = (jx + k.half_force[0]) * ir;
+    const C uy = (jy + k.half_force[1]) * ir;
+    const C uz = (jz + k.half_force[2]) * ir;
+    const C m3uF = (C)-3 * (ux * k.force[0] + uy * k.force[1] + uz * k.force[2]);
+    std_relax<Q, S, NT, true>(k, gv, rho, ux, uy, uz, m3uF, ld.solid, out0);
+  } else {
+    std_relax<Q, S, NT, false>(k, gv, rho, jx * ir, jy * ir, jz * ir, (C)0, ld.solid, out0);
   }
 }
 

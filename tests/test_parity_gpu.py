@@ -404,12 +404,16 @@ def test_c2_full_size_mask_and_mass(storage, tol):
     assert c1[1] > 0  # net momentum along the force
 
 
-def test_std_f64_matches_oracle_f64():
-    say("case: FP64 GPU vs FP64 oracle 24x20x16 x 200")
-    dims, q = (24, 20, 16), 19
+# with and without a body force (the collision skips the Guo terms of
+# lattices without one)
+@pytest.mark.parametrize("force", [(1e-5, 0, 0), (0, 0, 0)])
+@pytest.mark.parametrize("q,dims", [(19, (24, 20, 16)), (27, (20, 18, 14)),
+                                    (9, (40, 36, 1))])
+def test_std_f64_matches_oracle_f64(q, dims, force):
+    say(f"case: FP64 GPU vs FP64 oracle Q{q} {dims} force {force} x 200")
     rng = np.random.default_rng(4)
     solid = (rng.random(int(np.prod(dims))) < 0.1).astype(np.uint8)
-    kw = dict(tau=0.7, periodic=(1, 0, 0), force=(1e-5, 0, 0), lid_face=3,
+    kw = dict(tau=0.7, periodic=(1, 0, 0), force=force, lid_face=3,
               lid_u=(0.05, 0, 0))
     with _std_ctx(q, dims, storage=capi.STORE_F64, **kw) as c:
         c.upload_solid(solid)
@@ -417,7 +421,7 @@ def test_std_f64_matches_oracle_f64():
         c.step(200)
         fg = c.populations(np.float64)
     mask = O.std_build_mask(q, dims, solid, (1, 0, 0), 3)
-    p = O.std_params(tau=0.7, force=(1e-5, 0, 0), lid_face=3,
+    p = O.std_params(tau=0.7, force=force, lid_face=3,
                      lid_u=(0.05, 0, 0), periodic=(1, 0, 0))
     fo = O.std_run_f64(q, dims, O.std_init_eq(q, dims), mask, p, 200)
     assert np.abs(fg - fo).max() < 1e-12

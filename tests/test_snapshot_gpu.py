@@ -100,6 +100,14 @@ def test_snapshot_rejects_other_lattices(tmp_path):
         with pytest.raises(capi.LbmError):
             b.load(path)
         assert np.array_equal(b.populations(), before)
+    # same extents, other boundaries (a moving wall)
+    with capi.Context(capi.make_config(16, 16, 16, lattice=19, lid_face=3,
+                                       lid_u=(0.05, 0, 0))) as d:
+        d.init_equilibrium()
+        before = d.populations()
+        with pytest.raises(capi.LbmError):
+            d.load(path)
+        assert np.array_equal(d.populations(), before)
     with open(path, "r+b") as fh:  # truncated file
         fh.truncate(200)
     with capi.Context(capi.make_config(16, 16, 16, lattice=19)) as c:

@@ -51,6 +51,29 @@ def test_error_reporting_without_gpu_work():
     assert lib.lbm_step(None, 1) == 1
 
 
+def test_size_limits_and_null_arguments_without_gpu_work():
+    lib = capi.load()
+    h = capi.CTX()
+    # every extent below 2^24 (24-bit coordinate multiplies)
+    cfg = capi.make_config(1 << 24, 1, 1, lattice=19)
+    assert lib.lbm_create(C.byref(cfg), None, C.byref(h)) == 5
+    # population element indices below 2^31: 1024 x 1024 x 2048 x 19 is over
+    cfg = capi.make_config(1024, 1024, 2048, lattice=19)
+    assert lib.lbm_create(C.byref(cfg), None, C.byref(h)) == 5
+    assert b"32-bit" in lib.lbm_last_error()
+    # entry points of the snapshot and mesh paths reject null handles
+    assert lib.lbm_snapshot_save(None, b"/tmp/x") == 1
+    assert lib.lbm_snapshot_load(None, b"/tmp/x") == 1
+    inside = np.zeros(8, np.uint8)
+    cnt = C.c_int64()
+    assert lib.lbm_voxelize_mesh(None, None, 0, None, 0, C.c_float(1.0),
+                                 inside.ctypes.data_as(C.c_void_p),
+                                 C.byref(cnt)) == 1
+    n32 = C.c_int32()
+    assert lib.lbm_build_reflections(None, None, 0, None, 0, C.c_float(1.0),
+                                     0, None, None, None, C.byref(n32)) == 1
+
+
 @pytest.mark.parametrize("dims,seed,px", [((64, 32, 32), 2, True),
                                           ((40, 24, 20), 7, False),
                                           ((96, 48, 40), 2, True)])

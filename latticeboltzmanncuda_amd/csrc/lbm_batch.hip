@@ -335,6 +335,7 @@ int lbm_batch_add(lbm_batch* b, const lbm_config* cfg, const uint8_t* solid,
   lbm_ctx* c = nullptr;
   int rc = lbm_create(cfg, b->stream, &c);
   if (rc) return rc;
+  c->batch_member = true;
   if (solid && (rc = lbm_upload_solid(c, solid))) {
     lbm_destroy(c);
     return rc;

@@ -459,6 +459,9 @@ int step_reference_staged(lbm_ctx* c, int n, int with_force) {
 }
 
 }  // namespace
+
+int check_config(const lbm_config* cfg) { return validate_config(cfg); }
+
 }  // namespace lbm
 
 using namespace lbm;

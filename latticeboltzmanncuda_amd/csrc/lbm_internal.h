@@ -78,6 +78,9 @@ struct FlowSpec {
 };
 
 StdParams make_std_params(const lbm_config& cfg);
+// The checks lbm_create applies to a configuration (LBM_OK or an error
+// code with lbm_last_error set).
+int check_config(const lbm_config* cfg);
 // device bytes lbm_create allocates for cfg (populations, geometry, scratch)
 int64_t estimate_ctx_bytes(const lbm_config& cfg);
 
@@ -166,6 +169,7 @@ struct lbm_ctx {
   uint32_t* cls = nullptr;   // standard mode: 2-bit node classes
   uint64_t steps_done = 0;   // parity picks the tile order of the next step
   uint64_t params_version = 0;  // bumped when tau or the body force change
+  bool batch_member = false;    // owned by a batch (its kernel class is fixed)
   uint8_t* flags = nullptr;  // reference mode
   std::vector<uint8_t> host_flags;
 

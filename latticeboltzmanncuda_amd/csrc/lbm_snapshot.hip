@@ -156,16 +156,16 @@ int lbm_snapshot_load(lbm_ctx* c, const char* path) {
       h.header_bytes != sizeof(SnapHeader))
     return fail(LBM_ERR_INVALID, "not a snapshot of this library version");
   const lbm_config& s = h.cfg;
-  // the boundary setup (periodic axes, moving-wall face) must match too:
-  // it fixes the geometry tables and, for a batch member, its kernel class
   if (s.nx != c->cfg.nx || s.ny != c->cfg.ny || s.nz != c->cfg.nz || s.lattice != c->cfg.lattice ||
-      s.storage != c->cfg.storage || s.mode != c->cfg.mode || s.lid_face != c->cfg.lid_face ||
-      s.periodic[0] != c->cfg.periodic[0] || s.periodic[1] != c->cfg.periodic[1] ||
-      s.periodic[2] != c->cfg.periodic[2])
+      s.storage != c->cfg.storage || s.mode != c->cfg.mode)
     return fail(LBM_ERR_INVALID,
-                "snapshot extents / lattice / storage / mode / boundaries differ from the context");
-  if (!(s.tau > 0.0f) || (s.mode == LBM_MODE_STANDARD && !(s.tau > 0.5f)))
-    return fail(LBM_ERR_INVALID, "snapshot tau out of range");
+                "snapshot extents / lattice / storage / mode differ from the context");
+  // the snapshot's parameters replace the context's and pass the checks a
+  // new context's would (the geometry tables come with the snapshot)
+  if (int e = check_config(&s)) return e;
+  // a batch member's kernel class is fixed by whether it has a moving wall
+  if (c->batch_member && s.lid_face != c->cfg.lid_face)
+    return fail(LBM_ERR_INVALID, "a batch member's moving wall cannot change");
   const uint64_t N = c->grid.n;
   const bool standard = c->cfg.mode == LBM_MODE_STANDARD;
   if (h.pop_bytes != population_elems(c->q, N) * storage_bytes(c->cfg.storage) ||
@@ -203,7 +203,9 @@ int lbm_snapshot_load(lbm_ctx* c, const char* path) {
   for (int a = 0; a < 3; ++a) {
     c->cfg.force[a] = s.force[a];
     c->cfg.lid_u[a] = s.lid_u[a];
+    c->cfg.periodic[a] = s.periodic[a];
   }
+  c->cfg.lid_face = s.lid_face;
   c->cfg.wall_rho = s.wall_rho;
   ++c->params_version;
   c->steps_done = h.steps_done;

@@ -100,14 +100,13 @@ def test_snapshot_rejects_other_lattices(tmp_path):
         with pytest.raises(capi.LbmError):
             b.load(path)
         assert np.array_equal(b.populations(), before)
-    # same extents, other boundaries (a moving wall)
-    with capi.Context(capi.make_config(16, 16, 16, lattice=19, lid_face=3,
-                                       lid_u=(0.05, 0, 0))) as d:
-        d.init_equilibrium()
-        before = d.populations()
+    # a batch member cannot take a snapshot with another moving wall (its
+    # kernel class is fixed by having one)
+    with capi.Batch(capi.STORE_F32, 1 << 30) as bt:
+        m = bt.add(capi.make_config(16, 16, 16, lattice=19, lid_face=3,
+                                    lid_u=(0.05, 0, 0)))
         with pytest.raises(capi.LbmError):
-            d.load(path)
-        assert np.array_equal(d.populations(), before)
+            m.load(path)
     with open(path, "r+b") as fh:  # truncated file
         fh.truncate(200)
     with capi.Context(capi.make_config(16, 16, 16, lattice=19)) as c:

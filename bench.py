@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--c4-launch", default="",
                     help="block,grid of the batch kernel instead of the "
                          "autotuner's choice")
+    ap.add_argument("--c4-chunk", type=int, default=100,
+                    help="steps per hipGraph replay of the batch (0: plain "
+                         "launches, e.g. under a kernel tracer)")
     ap.add_argument("--c5-requests", type=int, default=64)
     ap.add_argument("--c5-lambda", default="100",
                     help="comma list of arrival rates (requests/s)")
@@ -85,14 +88,15 @@ def precision_section(steps):
             "runs": runs}
 
 
-def batched_section(steps, launch=None):
+def batched_section(steps, launch=None, chunk=100):
     """512 D2Q9 cavity/channel lattices (seed 4): plain per-lattice
-    launches vs one persistent launch per step in hipGraph chunks of 100,
-    FP32 and FP16 storage; FP16 velocity error against FP64 storage."""
+    launches vs one persistent launch per step, replayed in hipGraph chunks
+    of `chunk` steps (0: launched one by one), FP32 and FP16 storage; FP16
+    velocity error against FP64 storage."""
     from latticeboltzmanncuda_amd import capi, workloads
     out = {"workload": "512 D2Q9 lattices (seed 4), sides log-uniform "
                        "32-512, cavity/channel, Re 100-1000, tau >= 0.51, "
-                       "10% squares", "steps": steps, "chunk": 100}
+                       "10% squares", "steps": steps, "chunk": chunk}
     specs32, clamped = workloads.c4_specs(512, storage=capi.STORE_F32)
     out["tau_clamped"] = clamped
     out["nodes"] = int(sum(c.nx * c.ny for c, _, _ in specs32))
@@ -103,7 +107,7 @@ def batched_section(steps, launch=None):
         log(f"batched: {name} plain")
         plain = workloads.c4_plain_mlups(specs, steps)
         log(f"batched: {name} persistent")
-        bat = workloads.c4_batched_mlups(specs, steps, 100, True, st, launch)
+        bat = workloads.c4_batched_mlups(specs, steps, chunk, True, st, launch)
         out[name] = {
             "plain_mlups": round(plain["mlups"], 1),
             "batched_mlups": round(bat["mlups"], 1),
@@ -113,7 +117,7 @@ def batched_section(steps, launch=None):
             "launch": {"block": bat["block"], "grid": bat["grid"]},
         }
     log("batched: f16 vs f64 velocity error")
-    err = workloads.c4_velocity_error(specs32, capi.STORE_F16, 1000)
+    err = workloads.c4_velocity_error(specs32, capi.STORE_F16, 1000, chunk)
     out["f16"]["error_vs_f64"] = {
         k: (round(v, 8) if isinstance(v, float) else v)
         for k, v in err.items()}
@@ -353,7 +357,7 @@ def main():
         log("batched")
         launch = (tuple(int(v) for v in args.c4_launch.split(","))
                   if args.c4_launch else None)
-        out["batched"] = batched_section(args.c4_steps, launch)
+        out["batched"] = batched_section(args.c4_steps, launch, args.c4_chunk)
     if "reference" in sections:
         log("reference mode")
         out["reference_mode"] = reference_mode_section(args.ref_steps)

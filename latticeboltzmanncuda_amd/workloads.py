@@ -246,7 +246,7 @@ def c4_batched_mlups(specs, steps: int, chunk: int = 100,
             block, grid = launch
         else:
             block, grid = b.autotune(steps=10) if autotune else b.launch()
-        b.step(chunk, chunk)  # capture + warm-up
+        b.step(chunk or 10, chunk)  # warm-up (and capture, with chunks)
         b.synchronize()
         t0 = time.perf_counter()
         ms = b.time_steps(steps, chunk)

@@ -159,6 +159,10 @@ def serving_section(nreq, lams, max_steps, engines, node_cap):
                 "peak_gb": round(st["peak_bytes"] / 1e9, 3),
             }
         out["rates"][str(lam)] = row
+    log("serving: fp16 vs fp64 velocity error")
+    err = workloads.c5_velocity_error(16, 2000)
+    out["f16_error_vs_f64"] = {
+        k: (round(v, 8) if isinstance(v, float) else v) for k, v in err.items()}
     return out
 
 

@@ -127,21 +127,27 @@ __device__ __forceinline__ void std_gather(const Grid& g,
 
 // BGK relaxation towards the equilibrium of (rho, u), plus the Guo source
 // term when FORCED, and the stores of the Q outputs (solid nodes: the rest
-// state).
+// state).  rs is rho (full storage) or drho = rho - 1 (shifted storage).
+// Shifted storage forms feq - f = w rho t - w - g as
+//   w ((t - 1) + drho + drho (t - 1)) - g
+// from small terms only: no intermediate is near 1 or near w, so the
+// rounding error of the collision scales with the deviations and a body
+// force's small per-step increments are not lost.
 template <int Q, int S, bool NT, bool FORCED>
 __device__ __forceinline__ void std_relax(const StepCoef<typename Store<S>::C>& k,
                                           const typename Store<S>::C (&gv)[Q],
-                                          typename Store<S>::C rho, typename Store<S>::C ux,
+                                          typename Store<S>::C rs, typename Store<S>::C ux,
                                           typename Store<S>::C uy, typename Store<S>::C uz,
                                           typename Store<S>::C m3uF, bool solid,
-                                          typename Store<S>::T* out0) {
+                                          typename Store<S>::T* out0, uint32_t node_hash) {
 #pragma clang fp contract(on)
   using L = Lat<Q>;
   using St = Store<S>;
   using C = typename St::C;
   constexpr bool kSh = St::kShifted;
   const C usq = ux * ux + uy * uy + uz * uz;
-  const C base = (C)1 - (C)1.5 * usq;
+  // t - 1 (shifted) or t = 1 - 1.5 u.u + 3 e.u + 4.5 (e.u)^2 (full)
+  const C base = kSh ? (C)-1.5 * usq : (C)1 - (C)1.5 * usq;
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     // e.u from the nonzero components only (-0 + v == v exactly)
@@ -150,10 +156,10 @@ __device__ __forceinline__ void std_relax(const StepCoef<typename Store<S>::C>& 
     if (L::E[i][1]) eu = L::E[i][1] > 0 ? eu + uy : eu - uy;
     if (L::E[i][2]) eu = L::E[i][2] > 0 ? eu + uz : eu - uz;
     const C w = (C)L::W[i];
-    // t = 1 - 1.5 u.u + 3 e.u + 4.5 (e.u)^2, feq = w rho t
     const C t = fmadd(fmadd((C)4.5, eu, (C)3), eu, base);
-    // feq - f in the stored form: w rho t - w - g (shifted), w rho t - f
-    const C d = (kSh ? fmadd(w * rho, t, -w) : (w * rho) * t) - gv[i];
+    // feq - f in the stored form: w ((t-1) + drho + drho (t-1)) - g
+    // (shifted, t here is t - 1), w rho t - f (full)
+    const C d = kSh ? fmadd(w, fmadd(rs, t, t + rs), -gv[i]) : (w * rs) * t - gv[i];
     C out = fmadd(k.omega, d, gv[i]);
     if (FORCED) {
       // Guo source (1 - omega/2) w (3 (e.F - u.F) + 9 (e.u)(e.F)), added
@@ -163,7 +169,13 @@ __device__ __forceinline__ void std_relax(const StepCoef<typename Store<S>::C>& 
       out = fmadd(k.gf9[i], eu, out);
     }
     out = solid ? (kSh ? (C)0 : w) : out;
-    if (NT)
+    // a body force moves 16-bit populations by less than their rounding
+    // step: those lattices round stochastically (no effect at full precision)
+    if (FORCED && NT)
+      St::store_sr_nt(out0 + i * kTile, out, sr_key(node_hash, i));
+    else if (FORCED)
+      St::store_sr(out0 + i * kTile, out, sr_key(node_hash, i));
+    else if (NT)
       St::store_nt(out0 + i * kTile, out);
     else
       St::store(out0 + i * kTile, out);
@@ -199,16 +211,18 @@ __device__ __forceinline__ void std_collide(const StdParams& p,
     gv[i] = v;
   }
 
-  // moments; with shifted storage sum w_i = 1 and sum w_i e_i = 0
-  C rho = kSh ? (C)1 : (C)0, jx = 0, jy = 0, jz = 0;
+  // moments; with shifted storage sum w_i = 1 and sum w_i e_i = 0, so the
+  // sum of the stored values is rho itself (full storage) or the density
+  // deviation drho = rho - 1 (shifted storage)
+  C rs = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    rho += gv[i];
+    rs += gv[i];
     if (L::E[i][0]) jx += L::E[i][0] > 0 ? gv[i] : -gv[i];
     if (L::E[i][1]) jy += L::E[i][1] > 0 ? gv[i] : -gv[i];
     if (L::E[i][2]) jz += L::E[i][2] > 0 ? gv[i] : -gv[i];
   }
-  const C ir = recip(rho);
+  const C ir = recip(kSh ? (C)1 + rs : rs);
   // the node's Q outputs lie kTile elements apart: one 64-bit address, the
   // direction offsets become constant offsets of the stores
   typename St::T* const out0 = B + ld.own;
@@ -219,9 +233,10 @@ __device__ __forceinline__ void std_collide(const StdParams& p,
     const C uy = (jy + k.half_force[1]) * ir;
     const C uz = (jz + k.half_force[2]) * ir;
     const C m3uF = (C)-3 * (ux * k.force[0] + uy * k.force[1] + uz * k.force[2]);
-    std_relax<Q, S, NT, true>(k, gv, rho, ux, uy, uz, m3uF, ld.solid, out0);
+    std_relax<Q, S, NT, true>(k, gv, rs, ux, uy, uz, m3uF, ld.solid, out0,
+                              kSh ? sr_node_hash(ld.own) : 0u);
   } else {
-    std_relax<Q, S, NT, false>(k, gv, rho, jx * ir, jy * ir, jz * ir, (C)0, ld.solid, out0);
+    std_relax<Q, S, NT, false>(k, gv, rs, jx * ir, jy * ir, jz * ir, (C)0, ld.solid, out0, 0u);
   }
 }
 

@@ -1,7 +1,17 @@
 // Population storage types.  F32/F64 store f_i itself; F16/BF16 store the
 // shifted deviation f_i - w_i (the rest-state value w_i carries most of the
-// magnitude, so the 16-bit mantissa spends its bits on the deviation).
-// Conversions to 16 bits round to nearest, ties to even.
+// magnitude, so the 16-bit mantissa spends its bits on the deviation).  F16
+// holds the deviation times 2^12 (exact both ways), so deviations down to
+// 1.5e-8 are normal FP16 numbers; |f_i - w_i| up to 16 fits.
+//
+// Conversions to 16 bits round to nearest, ties to even -- or, through
+// store_sr, stochastically: a flow driven by a small body force changes its
+// populations by less than half a 16-bit unit in the last place per step,
+// so rounding to nearest would return the old value every step and the
+// flow would not accelerate.  Stochastic rounding rounds up with the
+// probability given by the dropped bits, so the increments survive on
+// average.  Its random bits are a hash of the value, the node and the
+// direction (sr_bits), so a lattice's result depends only on its own state.
 //
 // store_nt writes with the non-temporal hint: the line goes to HBM without
 // being kept in L2 / the Infinity Cache.  For a lattice whose two
@@ -21,6 +31,19 @@
 
 namespace lbm {
 
+// Random bits for stochastic rounding of the float with bit pattern b:
+// key = node hash + direction constant (sr_key).  One 24-bit multiply mixes
+// the low 24 bits of the value with the key; callers take the top bits of
+// the product (taking low ones would let the compiler drop the 24-bit
+// operand mask and fall back to the quarter-rate 32-bit multiply).
+__device__ __forceinline__ uint32_t sr_bits(uint32_t b, uint32_t key) {
+  return __umul24(b ^ key, 0x2C1B3Du);
+}
+__device__ __forceinline__ uint32_t sr_node_hash(uint32_t own) { return own * 0x9E3779B1u; }
+__device__ __forceinline__ uint32_t sr_key(uint32_t node_hash, int i) {
+  return node_hash + (uint32_t)i * 0x85EBCA77u;
+}
+
 template <int S>
 struct Store;
 
@@ -34,6 +57,9 @@ struct Store<LBM_STORE_F32> {
   __device__ static C load(const T* p) { return gload(p); }
   __device__ static void store(T* p, C v) { gstore(p, v); }
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, v); }
+  // full-precision storage rounds to nearest
+  __device__ static void store_sr(T* p, C v, uint32_t) { gstore(p, v); }
+  __device__ static void store_sr_nt(T* p, C v, uint32_t) { gstore_nt(p, v); }
 };
 
 template <>
@@ -46,6 +72,9 @@ struct Store<LBM_STORE_F64> {
   __device__ static C load(const T* p) { return gload(p); }
   __device__ static void store(T* p, C v) { gstore(p, v); }
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, v); }
+  // full-precision storage rounds to nearest
+  __device__ static void store_sr(T* p, C v, uint32_t) { gstore(p, v); }
+  __device__ static void store_sr_nt(T* p, C v, uint32_t) { gstore_nt(p, v); }
 };
 
 template <>
@@ -53,11 +82,24 @@ struct Store<LBM_STORE_F16> {
   using T = _Float16;
   using C = float;
   static constexpr bool kShifted = true;
-  __device__ static C cvt(T v) { return (float)v; }
+  static constexpr float kScale = 4096.0f, kUnscale = 1.0f / 4096.0f;
+  __device__ static C cvt(T v) { return (float)v * kUnscale; }
   __device__ static T rest(int, double) { return (T)0; }
-  __device__ static C load(const T* p) { return (float)gload(p); }
-  __device__ static void store(T* p, C v) { gstore(p, (_Float16)v); }
-  __device__ static void store_nt(T* p, C v) { gstore_nt(p, (_Float16)v); }
+  __device__ static C load(const T* p) { return cvt(gload(p)); }
+  __device__ static T round(C v) { return (_Float16)(v * kScale); }
+  // the 13 bits FP32 -> FP16 drops get a random addend, then are cut; a
+  // result in the FP16 subnormal range (below 1.5e-8 unscaled) is rounded
+  // to nearest by the conversion
+  __device__ static T round_sr(C v, uint32_t key) {
+    const uint32_t b = __float_as_uint(v * kScale);
+    const uint32_t r = sr_bits(b, key) >> 19;  // 13 bits
+    // saturating add: a NaN's pattern cannot wrap round to a number
+    return (_Float16)__uint_as_float(__builtin_elementwise_add_sat(b, r) & 0xFFFFE000u);
+  }
+  __device__ static void store(T* p, C v) { gstore(p, round(v)); }
+  __device__ static void store_nt(T* p, C v) { gstore_nt(p, round(v)); }
+  __device__ static void store_sr(T* p, C v, uint32_t key) { gstore(p, round_sr(v, key)); }
+  __device__ static void store_sr_nt(T* p, C v, uint32_t key) { gstore_nt(p, round_sr(v, key)); }
 };
 
 template <>
@@ -73,8 +115,14 @@ struct Store<LBM_STORE_BF16> {
     u += 0x7fffu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
   }
+  __device__ static T round_sr(C v, uint32_t key) {
+    const uint32_t u = __float_as_uint(v);
+    return (uint16_t)(__builtin_elementwise_add_sat(u, sr_bits(u, key) >> 16) >> 16);
+  }
   __device__ static void store(T* p, C v) { gstore(p, round(v)); }
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, round(v)); }
+  __device__ static void store_sr(T* p, C v, uint32_t key) { gstore(p, round_sr(v, key)); }
+  __device__ static void store_sr_nt(T* p, C v, uint32_t key) { gstore_nt(p, round_sr(v, key)); }
 };
 
 inline size_t storage_bytes(int storage) {

@@ -844,6 +844,52 @@ float orc_round_bf16(float x) {
   return r;
 }
 
+/* 16-bit storage of the shifted deviation v = f_i - w_i, as the GPU library
+ * stores it: FP16 holds v * 2^12 (so deviations down to 1.5e-8 are normal
+ * binary16 numbers), BF16 holds v.  Lattices with a body force round
+ * stochastically: the bits the conversion drops get a random addend before
+ * they are cut, the addend a hash of the value's float pattern, of the
+ * node's element index in the GPU's blocked layout ((n / 64) * Q * 64 +
+ * n % 64) and of the direction -- the GPU's rule, so the emulation rounds
+ * with the same distribution (its floats can differ from the GPU's by the
+ * order of operations, so individual roundings can differ).  Returns the
+ * stored value, unscaled. */
+static uint32_t orc_sr_key(int64_t n, int q, int i) {
+  const uint32_t own = (uint32_t)((n >> 6) * (int64_t)q * 64 + (n & 63));
+  return own * 0x9E3779B1u + (uint32_t)i * 0x85EBCA77u;
+}
+
+static uint32_t orc_sat_add(uint32_t a, uint32_t b) {
+  const uint32_t s = a + b;
+  return s < a ? 0xffffffffu : s;
+}
+
+static float orc_store16(int storage, float v, int sr, uint32_t key) {
+  if (storage == 2) {
+    float y = v * 4096.0f;
+    if (sr) {
+      uint32_t b;
+      memcpy(&b, &y, 4);
+      /* low 32 bits of the 24-bit product; its top 13 bits */
+      const uint32_t r = (((b ^ key) & 0xffffffu) * 0x2C1B3Du) >> 19;
+      b = orc_sat_add(b, r) & 0xffffe000u;
+      memcpy(&y, &b, 4);
+    }
+    return orc_round_fp16(y) / 4096.0f;
+  }
+  if (storage == 3) {
+    if (!sr) return orc_round_bf16(v);
+    uint32_t b;
+    memcpy(&b, &v, 4);
+    const uint32_t r = (((b ^ key) & 0xffffffu) * 0x2C1B3Du) >> 16;
+    b = orc_sat_add(b, r) & 0xffff0000u;
+    float o;
+    memcpy(&o, &b, 4);
+    return o;
+  }
+  return v;
+}
+
 /* One pull step of the standard model for one node, float compute.
  *   f_i = A_i(x - e_i)                  open link (periodic wrap)
  *   f_i = A_inv(i)(x) + 6 w_i rho_w e_i.u_w   bounced link (wall term only
@@ -929,18 +975,103 @@ static void std_step_f32(int q, int nx, int ny, int nz, const float* A,
   double W[27];
   std_table(q, E, I, W);
   const int64_t N = (int64_t)nx * ny * nz;
-  float wsh[27];
-  for (int i = 0; i < q; ++i) wsh[i] = storage ? (float)W[i] : 0.0f;
-#define LOAD(ARR, IDX, DIR) ((ARR)[IDX] + wsh[DIR])
-#define STORE(ARR, IDX, DIR, VAL)                                      \
-  do {                                                                 \
-    const float v_ = (VAL) - wsh[DIR];                                 \
-    (ARR)[IDX] = storage == 2 ? orc_round_fp16(v_)                     \
-                              : (storage == 3 ? orc_round_bf16(v_) : v_); \
-  } while (0)
+#define LOAD(ARR, IDX, DIR) ((ARR)[IDX])
+#define STORE(ARR, IDX, DIR, VAL) ((ARR)[IDX] = (VAL))
   STD_STEP_BODY(float)
 #undef LOAD
 #undef STORE
+}
+
+/* The step of a lattice with shifted 16-bit storage, in the GPU library's
+ * shifted form: the stored g_i = f_i - w_i are collided as
+ *   drho = sum g, u = (sum e g + F/2) / (1 + drho),
+ *   tm1 = (4.5 e.u + 3) e.u - 1.5 u.u                          (t - 1),
+ *   g*_i = g_i + omega (w_i (drho tm1 + (tm1 + drho)) - g_i)
+ *                         (= w rho t - w - g for rho = 1 + drho)
+ *          [+ gf3_i + gw_i (-3 u.F) + gf9_i e.u   with a body force]
+ * (gw = (1 - omega/2) w, gf3 = 3 gw e.F, gf9 = 9 gw e.F), each operation
+ * rounded to float; solid nodes store the rest state g = 0.  Working on
+ * g and drho -- never on f or on numbers near 1 -- keeps the rounding error
+ * of the collision proportional to the deviations, so increments far below
+ * f survive. */
+static void std_step_shifted(int q, int nx, int ny, int nz, const float* A,
+                             float* B, const uint32_t* mask,
+                             const orc_std_params* p, int storage) {
+  int E[27][3], I[27];
+  double W[27];
+  std_table(q, E, I, W);
+  const int64_t N = (int64_t)nx * ny * nz;
+  const int dims[3] = {nx, ny, nz};
+  const float omega = 1.0f / p->tau;
+  const float guo = 1.0f - 0.5f * omega;
+  const float F[3] = {p->force[0], p->force[1], p->force[2]};
+  const int forced = F[0] != 0.0f || F[1] != 0.0f || F[2] != 0.0f;
+  const int lax = p->lid_face >= 0 ? p->lid_face / 2 : 0;
+  const int lsg = p->lid_face >= 0 ? (p->lid_face % 2 ? 1 : -1) : 0;
+  float w[27], lidt[27], gw[27], gf3[27], gf9[27];
+  for (int i = 0; i < q; ++i) {
+    w[i] = (float)W[i];
+    lidt[i] = 6.0f * w[i] * p->wall_rho *
+              ((float)E[i][0] * p->lid_u[0] + (float)E[i][1] * p->lid_u[1] +
+               (float)E[i][2] * p->lid_u[2]);
+    const float eF = (float)E[i][0] * F[0] + (float)E[i][1] * F[1] + (float)E[i][2] * F[2];
+    gw[i] = guo * w[i];
+    gf3[i] = gw[i] * (3.0f * eF);
+    gf9[i] = gw[i] * (9.0f * eF);
+  }
+#pragma omp parallel for schedule(static)
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z) {
+        const int64_t n = idx3(x, y, z, ny, nz);
+        const uint32_t m = mask[n];
+        if (m & ORC_MASK_SOLID) {
+          for (int i = 0; i < q; ++i) B[i * N + n] = 0.0f;
+          continue;
+        }
+        const int c[3] = {x, y, z};
+        float g[27];
+        for (int i = 0; i < q; ++i) {
+          if ((m >> i) & 1u) {
+            g[i] = A[I[i] * N + n];
+            if ((m & ORC_MASK_LID) && E[i][lax] == -lsg) g[i] += lidt[i];
+          } else {
+            int s3[3];
+            for (int a = 0; a < 3; ++a) {
+              s3[a] = c[a] - E[i][a];
+              if (s3[a] < 0) s3[a] += dims[a];
+              if (s3[a] >= dims[a]) s3[a] -= dims[a];
+            }
+            g[i] = A[i * N + idx3(s3[0], s3[1], s3[2], ny, nz)];
+          }
+        }
+        float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+        for (int i = 0; i < q; ++i) {
+          drho += g[i];
+          jx += (float)E[i][0] * g[i];
+          jy += (float)E[i][1] * g[i];
+          jz += (float)E[i][2] * g[i];
+        }
+        const float ir = 1.0f / (1.0f + drho);
+        const float ux = (jx + 0.5f * F[0]) * ir, uy = (jy + 0.5f * F[1]) * ir,
+                    uz = (jz + 0.5f * F[2]) * ir;
+        const float m15usq = -1.5f * (ux * ux + uy * uy + uz * uz);
+        const float m3uF = -3.0f * (ux * F[0] + uy * F[1] + uz * F[2]);
+        for (int i = 0; i < q; ++i) {
+          const float eu = (float)E[i][0] * ux + (float)E[i][1] * uy + (float)E[i][2] * uz;
+          /* t - 1, and w rho t - w = w ((t - 1) + drho + drho (t - 1)), all
+           * formed from small terms */
+          const float tm1 = (4.5f * eu + 3.0f) * eu + m15usq;
+          const float d = w[i] * (drho * tm1 + (tm1 + drho)) - g[i];
+          float out = omega * d + g[i];
+          if (forced) {
+            out = out + gf3[i];
+            out = gw[i] * m3uF + out;
+            out = gf9[i] * eu + out;
+          }
+          B[i * N + n] = orc_store16(storage, out, forced, forced ? orc_sr_key(n, q, i) : 0u);
+        }
+      }
 }
 
 static void std_step_f64(int q, int nx, int ny, int nz, const double* A,
@@ -971,13 +1102,14 @@ void orc_std_run_f32(int q, int nx, int ny, int nz, float* fA,
   for (int i = 0; i < q; ++i)
     for (int64_t n = 0; n < N; ++n) {
       const float v = fA[i * N + n];
-      A[i * N + n] = storage == 2   ? orc_round_fp16(v - (float)W[i])
-                     : storage == 3 ? orc_round_bf16(v - (float)W[i])
-                                    : v;
+      A[i * N + n] = storage ? orc_store16(storage, v - (float)W[i], 0, 0u) : v;
     }
   memcpy(B, A, bytes);
   for (int s = 0; s < steps; ++s) {
-    std_step_f32(q, nx, ny, nz, A, B, mask, p, storage);
+    if (storage)
+      std_step_shifted(q, nx, ny, nz, A, B, mask, p, storage);
+    else
+      std_step_f32(q, nx, ny, nz, A, B, mask, p, 0);
     float* t = A;
     A = B;
     B = t;

@@ -523,3 +523,26 @@ def test_std_f32_tracks_f64():
     ob16 = O.std_run_f32(q, dims, f, mask, p, 50, O.STORE_BF16_SHIFTED)
     assert np.abs(o16 - o64).max() < 1e-4
     assert np.abs(ob16 - o64).max() < 2e-3
+
+
+def test_std_fp16_forced_channel_does_not_stall():
+    # a channel spun up from rest by a small body force: each step moves the
+    # stored deviations by ~3 w F (1 - omega/2) ~ 3e-9, far below half a
+    # binary16 unit in the last place of values near 1e-5.  Rounding to
+    # nearest would return the old value every step; the stochastic rounding
+    # of forced lattices (with the 2^12 scale) keeps the velocity within
+    # 1e-3 of the FP64 run.
+    q, dims = 9, (8, 64, 1)
+    mask = O.std_build_mask(q, dims, None, (1, 0, 0))
+    p = O.std_params(tau=0.64, force=(5e-8, 0, 0), periodic=(1, 0, 0))
+    f = O.std_init_eq(q, dims)
+    o64 = O.std_run_f64(q, dims, f, mask, p, 2000)
+    o16 = O.std_run_f32(q, dims, f, mask, p, 2000, O.STORE_F16_SHIFTED)
+    _, u64 = O.std_moments(q, dims, o64, mask, (5e-8, 0, 0))
+    _, u16 = O.std_moments(q, dims, o16, mask, (5e-8, 0, 0))
+    err = np.sqrt(((u16 - u64) ** 2).sum() / (u64 ** 2).sum())
+    assert err < 1e-3
+    # BF16 (8-bit mantissa) also tracks the flow instead of stalling
+    ob = O.std_run_f32(q, dims, f, mask, p, 2000, O.STORE_BF16_SHIFTED)
+    _, ub = O.std_moments(q, dims, ob, mask, (5e-8, 0, 0))
+    assert np.sqrt(((ub - u64) ** 2).sum() / (u64 ** 2).sum()) < 2e-2

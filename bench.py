@@ -229,10 +229,12 @@ def cpu_baseline(solid: np.ndarray, seconds: float) -> dict:
 
     def timed(budget, nthreads):
         O.set_threads(nthreads)
-        f = O.std_run_f32(q, DIMS, f0, mask, p, 1)  # warm-up step
+        # the GPU kernel's arithmetic: FP32 f_i - w_i, shifted collision
+        st = O.STORE_F32_SHIFTED
+        f = O.std_run_f32(q, DIMS, f0, mask, p, 1, st)  # warm-up step
         steps, t0 = 0, time.perf_counter()
         while time.perf_counter() - t0 < budget:
-            f = O.std_run_f32(q, DIMS, f, mask, p, chunk)
+            f = O.std_run_f32(q, DIMS, f, mask, p, chunk, st)
             steps += chunk
         return steps, time.perf_counter() - t0
 
@@ -252,7 +254,7 @@ def cpu_baseline(solid: np.ndarray, seconds: float) -> dict:
         "unit": "MLUPS",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle FP32 standard-mode step (C, OpenMP {threads} "
+        "sample": (f"oracle FP32 (shifted) standard-mode step (C, OpenMP {threads} "
                    f"threads) on the same 256x128x128 sphere channel, "
                    f"{steps} steps in {dt:.1f} s; includes a per-call "
                    f"population copy-in/out every {chunk} steps"),

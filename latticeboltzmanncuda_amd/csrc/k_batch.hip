@@ -170,6 +170,10 @@ int launch_batch_residual(int storage, const BatchLattice* lats,
     case LBM_STORE_BF16:
       hipLaunchKernelGGL(k_batch_residual<LBM_STORE_BF16>, g, b, 0, s, lats, chunks, nchunks, partials);
       break;
+    case kStoreF32Shifted:
+      hipLaunchKernelGGL(k_batch_residual<kStoreF32Shifted>, g, b, 0, s, lats, chunks, nchunks,
+                         partials);
+      break;
     default:
       return fail(LBM_ERR_UNSUPPORTED, "unknown storage");
   }

@@ -176,7 +176,8 @@ int launch_steps(lbm_batch* b, int n) {
     for (int k = 0; k < kBatchClasses; ++k) {
       const int grid = class_grid(b, k);
       if (grid == 0) continue;
-      int rc = launch_batch_step(b->storage, k, b->nt, b->block, grid, b->d_lats, b->d_tiles[k],
+      int rc = launch_batch_step(device_storage(LBM_MODE_STANDARD, b->storage), k, b->nt,
+                                 b->block, grid, b->d_lats, b->d_tiles[k],
                                  b->d_ntiles + k, s & 1, b->stream);
       if (rc) return rc;
     }
@@ -544,7 +545,8 @@ int lbm_batch_residuals(lbm_batch* b, double* out, int32_t nslots) {
   if (rc) return rc;
   for (int32_t s = 0; s < nslots; ++s) out[s] = -1.0;
   if (b->nchunks == 0) return LBM_OK;
-  if ((rc = launch_batch_residual(b->storage, b->d_lats, b->d_chunks, b->nchunks,
+  if ((rc = launch_batch_residual(device_storage(LBM_MODE_STANDARD, b->storage), b->d_lats,
+                                  b->d_chunks, b->nchunks,
                                   b->d_partials, b->stream)))
     return rc;
   std::vector<double> part(2 * (size_t)b->nchunks);

@@ -390,7 +390,7 @@ int rebuild_special(lbm_ctx* c) {
 int step_standard(lbm_ctx* c, int n) {
   const StdParams p = make_std_params(c->cfg);
   for (int s = 0; s < n; ++s) {
-    int rc = launch_std_step(c->q, c->cfg.storage, c->grid, c->f[c->cur],
+    int rc = launch_std_step(c->q, device_storage(c->cfg.mode, c->cfg.storage), c->grid, c->f[c->cur],
                              c->f[c->cur ^ 1], c->mask, c->cls, p,
                              (int)(c->steps_done & 1), c->stream);
     if (rc) return rc;
@@ -623,7 +623,7 @@ int lbm_upload_points(lbm_ctx* c, const lbm_lattice_point* pts, int64_t count) {
   if ((rc = st.alloc(bytes))) return rc;
   if ((rc = h2d(c, st.p, pts, bytes))) return rc;
   const bool ref = c->cfg.mode == LBM_MODE_REFERENCE;
-  if ((rc = launch_points_in(c->q, c->cfg.storage, c->grid,
+  if ((rc = launch_points_in(c->q, device_storage(c->cfg.mode, c->cfg.storage), c->grid,
                              (const lbm_lattice_point*)st.p, c->f[c->cur],
                              ref ? c->flags : nullptr, c->stream)))
     return rc;
@@ -647,7 +647,7 @@ int lbm_download_points(lbm_ctx* c, lbm_lattice_point* pts, int64_t count) {
   const size_t bytes = sizeof(lbm_lattice_point) * (size_t)count;
   int rc;
   if ((rc = st.alloc(bytes))) return rc;
-  if ((rc = launch_points_out(c->q, c->cfg.storage, c->grid, c->f[c->cur],
+  if ((rc = launch_points_out(c->q, device_storage(c->cfg.mode, c->cfg.storage), c->grid, c->f[c->cur],
                               c->flags, (lbm_lattice_point*)st.p, c->stream)))
     return rc;
   return d2h(c, pts, st.p, bytes);
@@ -661,7 +661,7 @@ static int set_pops(lbm_ctx* c, const void* f, int is_f64) {
   int rc;
   if ((rc = st.alloc(bytes))) return rc;
   if ((rc = h2d(c, st.p, f, bytes))) return rc;
-  if ((rc = launch_convert_in(c->q, c->cfg.storage, c->grid, st.p, is_f64,
+  if ((rc = launch_convert_in(c->q, device_storage(c->cfg.mode, c->cfg.storage), c->grid, st.p, is_f64,
                               c->f[c->cur], c->stream)))
     return rc;
   if ((rc = mirror_buffers(c))) return rc;
@@ -675,7 +675,7 @@ static int get_pops(lbm_ctx* c, void* f, int is_f64) {
   DevBuf st;
   int rc;
   if ((rc = st.alloc(bytes))) return rc;
-  if ((rc = launch_convert_out(c->q, c->cfg.storage, c->grid, c->f[c->cur],
+  if ((rc = launch_convert_out(c->q, device_storage(c->cfg.mode, c->cfg.storage), c->grid, c->f[c->cur],
                                st.p, is_f64, c->stream)))
     return rc;
   return d2h(c, f, st.p, bytes);
@@ -699,7 +699,7 @@ int lbm_init_equilibrium(lbm_ctx* c, const double* rho, const double* u) {
     if ((rc = du.alloc(sizeof(double) * 3 * N))) return rc;
     if ((rc = h2d(c, du.p, u, sizeof(double) * 3 * N))) return rc;
   }
-  if ((rc = launch_init_eq(c->q, c->cfg.storage, c->grid, (const double*)dr.p,
+  if ((rc = launch_init_eq(c->q, device_storage(c->cfg.mode, c->cfg.storage), c->grid, (const double*)dr.p,
                            (const double*)du.p, c->f[c->cur], c->stream)))
     return rc;
   if ((rc = mirror_buffers(c))) return rc;
@@ -925,7 +925,7 @@ int lbm_moments(lbm_ctx* c, float* rho, float* u) {
     rc = launch_ref_moments(c->grid, (const float*)c->f[c->cur],
                             (float*)dr.p, (float*)du.p, c->stream);
   else
-    rc = launch_std_moments(c->q, c->cfg.storage, c->grid, c->f[c->cur],
+    rc = launch_std_moments(c->q, device_storage(c->cfg.mode, c->cfg.storage), c->grid, c->f[c->cur],
                             c->mask, c->cfg.force, (float*)dr.p, (float*)du.p,
                             c->stream);
   if (rc) return rc;
@@ -937,7 +937,7 @@ int lbm_moments(lbm_ctx* c, float* rho, float* u) {
 int lbm_conservation(lbm_ctx* c, double out[4]) {
   if (int rc = check_ctx(c)) return rc;
   if (!out) return fail(LBM_ERR_INVALID, "null output");
-  int rc = launch_conservation(c->q, c->cfg.storage, c->grid, c->f[c->cur],
+  int rc = launch_conservation(c->q, device_storage(c->cfg.mode, c->cfg.storage), c->grid, c->f[c->cur],
                                c->mask, c->red, c->red_cap, c->red_out,
                                c->stream);
   if (rc) return rc;

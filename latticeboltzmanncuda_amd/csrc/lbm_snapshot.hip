@@ -29,7 +29,7 @@ using namespace lbm;
 namespace {
 
 constexpr char kMagic[8] = {'L', 'B', 'M', 'S', 'N', 'A', 'P', '1'};
-constexpr uint32_t kVersion = 2;  // 2: FP16 populations stored times 2^12
+constexpr uint32_t kVersion = 2;  // 2: FP16 times 2^12, standard FP32 shifted
 
 struct SnapHeader {
   char magic[8];

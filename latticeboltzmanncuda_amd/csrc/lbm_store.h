@@ -62,6 +62,33 @@ struct Store<LBM_STORE_F32> {
   __device__ static void store_sr_nt(T* p, C v, uint32_t) { gstore_nt(p, v); }
 };
 
+// Standard-mode FP32 storage (internal code; the public storage type is
+// LBM_STORE_F32): the shifted deviation f_i - w_i in float, collided in
+// the shifted form like the 16-bit types.  Increments far below f_i (a
+// small body force) then keep their precision.  Reference-mode lattices
+// store f_i itself (Store<LBM_STORE_F32>), as the reference does.
+constexpr int kStoreF32Shifted = 4;
+
+template <>
+struct Store<kStoreF32Shifted> {
+  using T = float;
+  using C = float;
+  static constexpr bool kShifted = true;
+  __device__ static C cvt(T v) { return v; }
+  __device__ static T rest(int, double) { return 0.0f; }
+  __device__ static C load(const T* p) { return gload(p); }
+  __device__ static void store(T* p, C v) { gstore(p, v); }
+  __device__ static void store_nt(T* p, C v) { gstore_nt(p, v); }
+  __device__ static void store_sr(T* p, C v, uint32_t) { gstore(p, v); }
+  __device__ static void store_sr_nt(T* p, C v, uint32_t) { gstore_nt(p, v); }
+};
+
+// The storage code the kernels use for a lattice of this mode and public
+// storage type.
+inline int device_storage(int mode, int storage) {
+  return mode == LBM_MODE_STANDARD && storage == LBM_STORE_F32 ? kStoreF32Shifted : storage;
+}
+
 template <>
 struct Store<LBM_STORE_F64> {
   using T = double;
@@ -128,6 +155,7 @@ struct Store<LBM_STORE_BF16> {
 inline size_t storage_bytes(int storage) {
   switch (storage) {
     case LBM_STORE_F32:
+    case kStoreF32Shifted:
       return 4;
     case LBM_STORE_F64:
       return 8;
@@ -181,14 +209,17 @@ int dispatch_qs(int q, int storage, Fn&& fn) {
     case 9 * 8 + LBM_STORE_F64: fn(QS<9, LBM_STORE_F64>{}); return LBM_OK;
     case 9 * 8 + LBM_STORE_F16: fn(QS<9, LBM_STORE_F16>{}); return LBM_OK;
     case 9 * 8 + LBM_STORE_BF16: fn(QS<9, LBM_STORE_BF16>{}); return LBM_OK;
+    case 9 * 8 + kStoreF32Shifted: fn(QS<9, kStoreF32Shifted>{}); return LBM_OK;
     case 19 * 8 + LBM_STORE_F32: fn(QS<19, LBM_STORE_F32>{}); return LBM_OK;
     case 19 * 8 + LBM_STORE_F64: fn(QS<19, LBM_STORE_F64>{}); return LBM_OK;
     case 19 * 8 + LBM_STORE_F16: fn(QS<19, LBM_STORE_F16>{}); return LBM_OK;
     case 19 * 8 + LBM_STORE_BF16: fn(QS<19, LBM_STORE_BF16>{}); return LBM_OK;
+    case 19 * 8 + kStoreF32Shifted: fn(QS<19, kStoreF32Shifted>{}); return LBM_OK;
     case 27 * 8 + LBM_STORE_F32: fn(QS<27, LBM_STORE_F32>{}); return LBM_OK;
     case 27 * 8 + LBM_STORE_F64: fn(QS<27, LBM_STORE_F64>{}); return LBM_OK;
     case 27 * 8 + LBM_STORE_F16: fn(QS<27, LBM_STORE_F16>{}); return LBM_OK;
     case 27 * 8 + LBM_STORE_BF16: fn(QS<27, LBM_STORE_BF16>{}); return LBM_OK;
+    case 27 * 8 + kStoreF32Shifted: fn(QS<27, kStoreF32Shifted>{}); return LBM_OK;
     default:
       return fail(LBM_ERR_UNSUPPORTED, "lattice/storage combination not built");
   }

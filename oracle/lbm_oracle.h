@@ -166,9 +166,11 @@ typedef struct orc_std_params {
 } orc_std_params;
 void orc_std_init_eq_f64(int q, int nx, int ny, int nz, const double* rho,
                          const double* u, double* f);
-/* storage: 0 float, 2 fp16 shifted, 3 bf16 shifted (values f_i - w_i are
- * rounded to the storage type after every step).  fA is in/out (float).
- * Solid nodes are set to the rest state w_i by every step. */
+/* storage: 0 float f_i, computed on f_i; 2 fp16 shifted, 3 bf16 shifted,
+ * 4 float shifted: values f_i - w_i, computed in the shifted form and
+ * rounded to the storage type after every step (the GPU library's
+ * standard-mode FP32 is 4).  fA is in/out (float).  Solid nodes are set to
+ * the rest state w_i by every step. */
 void orc_std_run_f32(int q, int nx, int ny, int nz, float* fA,
                      const uint32_t* mask, const orc_std_params* p, int steps,
                      int storage);

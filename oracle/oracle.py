@@ -20,7 +20,7 @@ FLAG_REFLECTED = 2
 FLOW_NONE, FLOW_POINT, FLOW_PLANE, FLOW_BOX = 0, 1, 2, 3
 MASK_SOLID = 1 << 31
 MASK_LID = 1 << 30
-STORE_F32, STORE_F16_SHIFTED, STORE_BF16_SHIFTED = 0, 2, 3
+STORE_F32, STORE_F16_SHIFTED, STORE_BF16_SHIFTED, STORE_F32_SHIFTED = 0, 2, 3, 4
 
 
 def build() -> None:

@@ -31,7 +31,8 @@ def test_streaming_store_d3q19_f32_channel():
         fg = c.populations(np.float64)
     mask = O.std_build_mask(q, dims, solid, (1, 0, 0))
     p = O.std_params(tau=0.8, force=(1e-5, 0, 0), periodic=(1, 0, 0))
-    fo = O.std_run_f32(q, dims, O.std_init_eq(q, dims), mask, p, steps)
+    fo = O.std_run_f32(q, dims, O.std_init_eq(q, dims), mask, p, steps,
+                       O.STORE_F32_SHIFTED)
     err = np.abs(fg - fo).max() / np.abs(fo).max()
     say(f"  max rel diff {err:.2e}")
     assert err < 1e-6

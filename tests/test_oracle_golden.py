@@ -518,6 +518,9 @@ def test_std_f32_tracks_f64():
     o64 = O.std_run_f64(q, dims, f, mask, p, 50)
     o32 = O.std_run_f32(q, dims, f, mask, p, 50)
     assert np.abs(o32 - o64).max() < 1e-6
+    # the shifted form (f_i - w_i, collided from small terms) is closer still
+    o32s = O.std_run_f32(q, dims, f, mask, p, 50, O.STORE_F32_SHIFTED)
+    assert np.abs(o32s - o64).max() < 0.2 * np.abs(o32 - o64).max()
     # shifted fp16 storage stays close; bf16 is coarser
     o16 = O.std_run_f32(q, dims, f, mask, p, 50, O.STORE_F16_SHIFTED)
     ob16 = O.std_run_f32(q, dims, f, mask, p, 50, O.STORE_BF16_SHIFTED)

@@ -126,8 +126,7 @@ def test_serve_fp16_accuracy_vs_fp64():
         d = np.abs(a["u"].astype(np.float64) - b["u"].astype(np.float64))
         rel = np.sqrt((d ** 2).sum() / (b["u"].astype(np.float64) ** 2).sum())
         say(f"  fp16 vs fp64 rel L2 velocity error {rel:.2e}")
-        # 16-bit quantisation of f - w_i is an absolute error, so the
-        # relative error grows as the velocity shrinks: the slow D3Q19
-        # channel (|u| ~ 3e-3 after a few hundred steps) sits near 1.3e-3,
-        # the cavities (lid 0.05) near 4e-4
-        assert rel < 2e-3
+        # FP16 storage of (f - w_i) * 2^12, collided from small terms,
+        # stochastic rounding on the forced channels: 2-4e-4 measured
+        # (with round-to-nearest the slow D3Q19 channel sat near 1.3e-3)
+        assert rel < 1e-3

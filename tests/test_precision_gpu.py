@@ -58,3 +58,27 @@ def test_fp64_gpu_matches_fp64_oracle_tgv():
     fo = O.std_run_f64(19, dims, O.std_init_eq(19, dims, rho, u), mask, p,
                        steps)
     assert np.abs(fg - fo).max() < 1e-13
+
+
+@pytest.mark.parametrize("storage,tol", [(capi.STORE_F16, 1e-3),
+                                         (capi.STORE_BF16, 2e-2)])
+def test_slow_forced_channel_does_not_stall(storage, tol):
+    # a channel spun up from rest by a body force of 5e-8 per step moves its
+    # populations by ~3e-9 per step, below half a 16-bit unit in the last
+    # place: forced lattices round stochastically, so the 16-bit run follows
+    # the FP64 one instead of staying at rest
+    say(f"case: slow forced channel D2Q9 8x64 storage {storage} x 2000")
+    dims = (8, 64, 1)
+    kw = dict(tau=0.64, periodic=(1, 0, 0), force=(5e-8, 0, 0))
+    u = {}
+    for st in (storage, capi.STORE_F64):
+        cfg = capi.make_config(*dims, lattice=capi.D2Q9, storage=st, **kw)
+        with capi.Context(cfg) as c:
+            c.init_equilibrium()
+            c.step(2000)
+            u[st] = c.moments()[1].astype(np.float64)
+    a, b = u[storage], u[capi.STORE_F64]
+    err = np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum())
+    say(f"  |u| {np.abs(b).max():.2e}, rel err {err:.2e}")
+    assert np.abs(b).max() > 1e-5  # the FP64 flow is moving
+    assert err < tol

@@ -13,16 +13,21 @@
 //      nothing;
 //   3. collide (BGK + Guo forcing) in FP32 (FP64 for F64 storage);
 //   4. store every direction with all lanes active; solid nodes store the
-//      rest state w_i.  Skipping the stores of solid lanes leaves partly
-//      written cache lines that cost ~1.7x in time on MI355X.  NT selects
-//      non-temporal stores (lattices larger than the Infinity Cache).
+//      rest state (w_i, or 0 for shifted storage).  Skipping the stores of
+//      solid lanes leaves partly written cache lines that cost ~1.7x in
+//      time on MI355X.  NT selects non-temporal stores (lattices larger
+//      than the Infinity Cache).
+//
+// Shifted storage (FP32, FP16, BF16 in standard mode) holds
+// g_i = f_i - w_i and is collided in that form from small terms only
+// (drho = sum g_i, j = sum e_i g_i; see std_relax), so deviations and a
+// body force's small increments are not cancelled against w_i or 1.
 //
 // The FP16 / BF16 paths are bound by vector-instruction issue, not by HBM,
 // so the update is written to spend few instructions per node: per-lattice
 // constants come precomputed (StepCoef), addresses need no 32-bit integer
-// multiplications, and shifted storage is collided in shifted form
-// (g_i = f_i - w_i, rho = 1 + sum g_i, j = sum e_i g_i), which also keeps
-// the small deviations instead of cancelling them against w_i.
+// multiplications, the body-force terms run only for lattices that have
+// one, and solid lanes get no values.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -249,6 +249,7 @@ def cpu_baseline(solid: np.ndarray, seconds: float) -> dict:
                           if l.startswith("model name")), "")
     except OSError:
         pass
+    ref = reference_cpu_baseline(O, seconds / 4, threads)
     return {
         "value": round(n * steps / dt / 1e6, 2),
         "unit": "MLUPS",
@@ -262,7 +263,44 @@ def cpu_baseline(solid: np.ndarray, seconds: float) -> dict:
         "single_thread_sample": f"{steps1} steps in {dt1:.1f} s, 1 thread",
         "cpu_model": model,
         "nproc": os.cpu_count(),
+        "reference_main_loop": ref,
     }
+
+
+def reference_cpu_baseline(O, seconds, threads):
+    """The oracle's restatement of the reference's main loop (stream,
+    reflect with force, collide, plane injection; LatticeBoltzmannCuda.cpp
+    :85-93) on the reference-mode bench workload: D3Q27 128^3 with the
+    angled cube at spacing 0.0125, timed at all threads and at one."""
+    dims, spacing = (128, 128, 128), 0.0125
+    mesh = O.Mesh(os.path.join(ROOT, "tests", "golden", "dataFiles",
+                               "cube_angled.obj"))
+    ok, internal = mesh.insert(dims, spacing)
+    flags, recs = mesh.preprocess(dims, spacing, internal * O.FLAG_INTERNAL)
+    n = int(np.prod(dims))
+    flow = np.full(27, 0.001, np.float32)
+    flow[6] = 0.002
+    f = np.tile(flow[:, None], (1, n)).reshape(-1).copy()
+    force = np.zeros(3, np.float32)
+    out = {"workload": "D3Q27 FP32 128^3 reference main loop, angled cube "
+                       "(spacing 0.0125), plane injection, force on, tau 0.6",
+           "reflection_records": int(len(recs[0]))}
+    for key, nt, budget in (("all_threads", threads, seconds),
+                            ("single_thread", 1, seconds)):
+        O.set_threads(nt)
+        g = O.ref_run_main(dims, f, flags, recs, 0.6, O.FLOW_PLANE, [0, 0],
+                           flow, force, 1)  # warm-up step
+        steps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget:
+            g = O.ref_run_main(dims, g, flags, recs, 0.6, O.FLOW_PLANE,
+                               [0, 0], flow, force, 2)
+            steps += 2
+        dt = time.perf_counter() - t0
+        out[key] = {"mlups": round(n * steps / dt / 1e6, 2),
+                    "threads": nt, "steps": steps,
+                    "seconds": round(dt, 2)}
+    O.set_threads(threads)
+    return out
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles",

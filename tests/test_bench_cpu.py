@@ -54,3 +54,7 @@ def test_cpu_baseline_leg_runs(monkeypatch):
     assert np.isfinite(out["value"])
     assert out["single_thread_value"] > 0
     assert out["nproc"] >= 1
+    # the reference's main loop on the CPU (reference-mode workload)
+    ref = out["reference_main_loop"]
+    assert ref["reflection_records"] > 0
+    assert ref["all_threads"]["mlups"] > 0 and ref["single_thread"]["mlups"] > 0

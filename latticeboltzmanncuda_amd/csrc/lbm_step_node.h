@@ -255,22 +255,4 @@ __device__ __forceinline__ void std_node_step(
   std_collide<Q, S, LID, NT>(p, B, ld);
 }
 
-// Two nodes per thread: both nodes' loads are issued before either is
-// collided, doubling the bytes in flight per wavefront (16-bit storage
-// moves too few bytes per load instruction to cover HBM latency with one
-// node per thread).  Nodes past the lattice end (n >= N) load node N - 1
-// and store nothing.
-template <int Q, int S, bool LID, bool NT>
-__device__ __forceinline__ void std_node_step2(
-    const Grid& g, const typename Store<S>::T* __restrict__ A,
-    typename Store<S>::T* __restrict__ B, const uint32_t* __restrict__ mask,
-    const uint32_t* __restrict__ cls, const StdParams& p, uint32_t n0, uint32_t n1) {
-  NodeLoad<Q, S> a, b;
-  const uint32_t last = g.n - 1;
-  std_gather<Q, S>(g, A, mask, cls, n0 < g.n ? n0 : last, a);
-  std_gather<Q, S>(g, A, mask, cls, n1 < g.n ? n1 : last, b);
-  if (n0 < g.n) std_collide<Q, S, LID, NT>(p, B, a);
-  if (n1 < g.n) std_collide<Q, S, LID, NT>(p, B, b);
-}
-
 }  // namespace lbm

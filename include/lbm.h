@@ -209,7 +209,7 @@ int lbm_step(lbm_ctx* ctx, int n);
  * per lattice per step (the unbatched way to step many lattices; no
  * counterpart in the reference). */
 int lbm_step_each(lbm_ctx* const* ctxs, int32_t count, int n);
-/* Reference mode: n steps of the LatticeBoltzmannCuda.cpp:161-169 loop
+/* Reference mode: n steps of the LatticeBoltzmannCuda.cpp:85-93 loop
  * (stream, reflect, collide, flow). */
 int lbm_step_reference(lbm_ctx* ctx, int n, int with_force);
 /* hipEvent-timed n steps (lbm_step) on the context's stream. */

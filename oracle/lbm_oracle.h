@@ -17,7 +17,7 @@
  *                           simulation/source/Lattice.cpp:163-210
  *       orc_ref_flow        simulation/source/SimCalcFuncs.cu:200-215,
  *                           simulation/source/FlowCriterion.cu:5-16
- *       orc_ref_run_main    LatticeBoltzmannCuda.cpp:161-169 (S,R,C,F order)
+ *       orc_ref_run_main    LatticeBoltzmannCuda.cpp:85-93 (S,R,C,F order)
  *       orc_ref_run_lattice simulation/source/Lattice.cpp:222-226 (S then C)
  *   The force accumulator is kept twice: in float with one addition per
  *   term (the reference's atomicAdd semantics, SimCalcFuncs.cu:181-183) and
@@ -100,7 +100,7 @@ void orc_ref_reflect(int nx, int ny, int nz, const float* L, float* T,
                      double* force64 /* nullable, FP64 sum of the pass */);
 void orc_ref_flow(int nx, int ny, int nz, float* f, int kind,
                   const int32_t* params, const float* induced);
-/* Whole time loop of LatticeBoltzmannCuda.cpp:161-169: per step stream,
+/* Whole time loop of LatticeBoltzmannCuda.cpp:85-93: per step stream,
  * reflect (with force when force != NULL), collide, flow. */
 void orc_ref_run_main(int nx, int ny, int nz, float* f, const uint8_t* flags,
                       int nrec, const int32_t* rec_node,

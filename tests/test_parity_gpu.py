@@ -231,7 +231,7 @@ def _p1_setup(data_dir, dims, spacing, mesh_name):
                                            ("compat_cube.obj", 1.0),
                                            ("cube_angled.obj", 0.6)])
 def test_p1_reference_main_loop_1000_steps(data_dir, mesh_name, tau):
-    # LatticeBoltzmannCuda.cpp:29-169: f = 0.001 (f6 = 0.002), plane x = 0
+    # LatticeBoltzmannCuda.cpp:29-93: f = 0.001 (f6 = 0.002), plane x = 0
     # injection of the same vector each step, obstacle, force on.
     say(f"case: P1 reference main loop 32^3 {mesh_name} tau={tau} x 1000")
     dims, spacing, steps = (32, 32, 32), 0.05, 1000

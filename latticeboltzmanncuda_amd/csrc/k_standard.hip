@@ -43,6 +43,18 @@ __global__ void __launch_bounds__(kBlock)
   std_node_step<Q, S, LID, NT>(g, A, B, mask, cls, p, n);
 }
 
+// Groups of kGroup nodes per thread (std_group_step).
+template <int Q, int S, int NV, bool LID, bool NT>
+__global__ void __launch_bounds__(kBlock)
+    k_std_stepv(Grid g, const typename Store<S>::T* __restrict__ A,
+                typename Store<S>::T* __restrict__ B, const uint32_t* __restrict__ mask,
+                const uint32_t* __restrict__ cls, StdParams p, int rev) {
+  const uint32_t blk = rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
+  const uint32_t n0 = (blk * kBlock + threadIdx.x) * (uint32_t)NV;
+  if (n0 >= g.n) return;  // the rows of active groups are whole
+  std_group_step<Q, S, NV, LID, NT>(g, A, B, mask, cls, p, n0);
+}
+
 template <int Q>
 __global__ void __launch_bounds__(kBlock)
     k_build_mask(Grid g, const uint8_t* __restrict__ solid, int px, int py,
@@ -282,6 +294,7 @@ __global__ void __launch_bounds__(kBlock)
 inline unsigned blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 
+
 }  // namespace
 
 int launch_std_step(int q, int storage, const Grid& g, const void* A, void* B,
@@ -294,6 +307,34 @@ int launch_std_step(int q, int storage, const Grid& g, const void* A, void* B,
   // between steps is fastest; without them, a fixed order is
   const int r = nt ? rev : 0;
   const dim3 grid(nb), block(kBlock);
+  if (group_step_applies(q, storage, g, nt)) {
+    const dim3 gridv((g.n / kGroup + kBlock - 1) / kBlock);
+    auto go = [&](auto tag) {
+      using Tag = decltype(tag);
+      using T = typename Store<Tag::S>::T;
+      const T* a = (const T*)A;
+      T* b = (T*)B;
+      constexpr int V = kGroup;
+      if (lid && nt)
+        hipLaunchKernelGGL((k_std_stepv<9, Tag::S, V, true, true>), gridv, block, 0, s, g, a, b,
+                           mask, cls, p, r);
+      else if (lid)
+        hipLaunchKernelGGL((k_std_stepv<9, Tag::S, V, true, false>), gridv, block, 0, s, g, a, b,
+                           mask, cls, p, r);
+      else if (nt)
+        hipLaunchKernelGGL((k_std_stepv<9, Tag::S, V, false, true>), gridv, block, 0, s, g, a, b,
+                           mask, cls, p, r);
+      else
+        hipLaunchKernelGGL((k_std_stepv<9, Tag::S, V, false, false>), gridv, block, 0, s, g, a,
+                           b, mask, cls, p, r);
+    };
+    if (storage == LBM_STORE_F16)
+      go(QS<9, LBM_STORE_F16>{});
+    else
+      go(QS<9, LBM_STORE_BF16>{});
+    LBM_HIP(hipGetLastError());
+    return LBM_OK;
+  }
   const int drc = dispatch_qs(q, storage, [&](auto tag) {
     using Tag = decltype(tag);
     using T = typename Store<Tag::S>::T;

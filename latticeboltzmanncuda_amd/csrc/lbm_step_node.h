@@ -1,5 +1,6 @@
 // The standard-mode update of one node, shared by the single-lattice step
-// kernel (k_standard.hip) and the batched persistent kernel (k_batch.hip).
+// kernel (k_standard.hip) and the batched persistent kernel (k_batch.hip),
+// and of a group of nodes per thread (single lattices).
 //
 // For node n of lattice g:
 //   1. read its class from the 2-bit class map: interior fluid, boundary
@@ -27,10 +28,15 @@
 // so the update is written to spend few instructions per node: per-lattice
 // constants come precomputed (StepCoef), addresses need no 32-bit integer
 // multiplications, the body-force terms run only for lattices that have
-// one, and solid lanes get no values.
+// one, and solid lanes get no values.  Large D2Q9 lattices with 16-bit
+// storage take the group step (std_group_step, below): two nodes per thread
+// share their loads, stores and address arithmetic, with the same per-node
+// arithmetic, so both steps give the same bytes.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <cstdlib>
 
 #include "lbm_global.h"
 #include "lbm_internal.h"
@@ -130,21 +136,26 @@ __device__ __forceinline__ void std_gather(const Grid& g,
   }
 }
 
+template <bool B>
+struct Forced {
+  static constexpr bool value = B;
+};
+
 // BGK relaxation towards the equilibrium of (rho, u), plus the Guo source
-// term when FORCED, and the stores of the Q outputs (solid nodes: the rest
-// state).  rs is rho (full storage) or drho = rho - 1 (shifted storage).
+// term when FORCED; emit(Forced<FORCED>{}, i, out) receives the Q outputs
+// in the stored form (solid nodes: the rest state).  rs is rho (full
+// storage) or drho = rho - 1 (shifted storage).
 // Shifted storage forms feq - f = w rho t - w - g as
 //   w ((t - 1) + drho + drho (t - 1)) - g
 // from small terms only: no intermediate is near 1 or near w, so the
 // rounding error of the collision scales with the deviations and a body
 // force's small per-step increments are not lost.
-template <int Q, int S, bool NT, bool FORCED>
+template <int Q, int S, bool FORCED, class Emit>
 __device__ __forceinline__ void std_relax(const StepCoef<typename Store<S>::C>& k,
                                           const typename Store<S>::C (&gv)[Q],
                                           typename Store<S>::C rs, typename Store<S>::C ux,
                                           typename Store<S>::C uy, typename Store<S>::C uz,
-                                          typename Store<S>::C m3uF, bool solid,
-                                          typename Store<S>::T* out0, uint32_t node_hash) {
+                                          typename Store<S>::C m3uF, bool solid, Emit&& emit) {
 #pragma clang fp contract(on)
   using L = Lat<Q>;
   using St = Store<S>;
@@ -174,46 +185,34 @@ __device__ __forceinline__ void std_relax(const StepCoef<typename Store<S>::C>& 
       out = fmadd(k.gf9[i], eu, out);
     }
     out = solid ? (kSh ? (C)0 : w) : out;
-    // a body force moves 16-bit populations by less than their rounding
-    // step: those lattices round stochastically (no effect at full precision)
-    if (FORCED && NT)
-      St::store_sr_nt(out0 + i * kTile, out, sr_key(node_hash, i));
-    else if (FORCED)
-      St::store_sr(out0 + i * kTile, out, sr_key(node_hash, i));
-    else if (NT)
-      St::store_nt(out0 + i * kTile, out);
-    else
-      St::store(out0 + i * kTile, out);
+    emit(Forced<FORCED>{}, i, out);
   }
 }
 
-// Phase 2: collide the loaded node and store its Q post-collision values.
-template <int Q, int S, bool LID, bool NT>
-__device__ __forceinline__ void std_collide(const StdParams& p,
-                                            typename Store<S>::T* __restrict__ B,
-                                            const NodeLoad<Q, S>& ld) {
+// The collision of one node from its Q pulled (or bounced) values in the
+// stored form, gv[i] (f_i, or f_i - w_i for shifted storage), its link
+// mask m and solid flag; emit receives the outputs (see std_relax).
+template <int Q, int S, bool LID, class Emit>
+__device__ __forceinline__ void std_collide_node(const StdParams& p,
+                                                 typename Store<S>::C (&gv)[Q], uint32_t m,
+                                                 bool solid, Emit&& emit) {
   // a * b + c is fused only where one source expression has that form, so
-  // every instantiation (single-lattice or batch, any block size, with or
-  // without a moving wall) rounds identically
+  // every instantiation (single-lattice or batch, one or several nodes per
+  // thread, with or without a moving wall) rounds identically
 #pragma clang fp contract(on)
   using L = Lat<Q>;
   using St = Store<S>;
   using C = typename St::C;
   constexpr bool kSh = St::kShifted;
   const StepCoef<C>& k = step_coef<C>(p);
-  const uint32_t m = ld.m;
-  // g[i]: the stored quantity (f_i, or f_i - w_i for shifted storage), plus
   // the moving-wall term of bounced links from beyond the moving face
-  C gv[Q];
   const bool lidnode = LID && (m & LBM_MASK_LID);
 #pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    C v = St::cvt(ld.raw[i]);
-    if (LID && i > 0) {  // the rest direction never crosses a face
+  for (int i = 1; i < Q; ++i) {  // the rest direction never crosses a face
+    if (LID) {
       const bool wall = lidnode && ((m >> i) & 1u);
-      v += wall ? k.lid[i] : (C)0;
+      gv[i] += wall ? k.lid[i] : (C)0;
     }
-    gv[i] = v;
   }
 
   // moments; with shifted storage sum w_i = 1 and sum w_i e_i = 0, so the
@@ -228,9 +227,6 @@ __device__ __forceinline__ void std_collide(const StdParams& p,
     if (L::E[i][2]) jz += L::E[i][2] > 0 ? gv[i] : -gv[i];
   }
   const C ir = recip(kSh ? (C)1 + rs : rs);
-  // the node's Q outputs lie kTile elements apart: one 64-bit address, the
-  // direction offsets become constant offsets of the stores
-  typename St::T* const out0 = B + ld.own;
   // Guo forcing only where the lattice has a body force (uniform per
   // lattice); without one, u = j / rho and the source term is zero
   if (k.forced) {
@@ -238,11 +234,42 @@ __device__ __forceinline__ void std_collide(const StdParams& p,
     const C uy = (jy + k.half_force[1]) * ir;
     const C uz = (jz + k.half_force[2]) * ir;
     const C m3uF = (C)-3 * (ux * k.force[0] + uy * k.force[1] + uz * k.force[2]);
-    std_relax<Q, S, NT, true>(k, gv, rs, ux, uy, uz, m3uF, ld.solid, out0,
-                              kSh ? sr_node_hash(ld.own) : 0u);
+    std_relax<Q, S, true>(k, gv, rs, ux, uy, uz, m3uF, solid, emit);
   } else {
-    std_relax<Q, S, NT, false>(k, gv, rs, jx * ir, jy * ir, jz * ir, (C)0, ld.solid, out0, 0u);
+    std_relax<Q, S, false>(k, gv, rs, jx * ir, jy * ir, jz * ir, (C)0, solid, emit);
   }
+}
+
+// Phase 2: collide the loaded node and store its Q post-collision values.
+template <int Q, int S, bool LID, bool NT>
+__device__ __forceinline__ void std_collide(const StdParams& p,
+                                            typename Store<S>::T* __restrict__ B,
+                                            const NodeLoad<Q, S>& ld) {
+  using St = Store<S>;
+  using C = typename St::C;
+  C gv[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) gv[i] = St::cvt(ld.raw[i]);
+  // the node's Q outputs lie kTile elements apart: one 64-bit address, the
+  // direction offsets become constant offsets of the stores
+  typename St::T* const out0 = B + ld.own;
+  std_collide_node<Q, S, LID>(p, gv, ld.m, ld.solid, [&](auto forced, int i, C out) {
+    // a body force moves 16-bit populations by less than their rounding
+    // step: those lattices round stochastically (no effect at full
+    // precision)
+    if constexpr (decltype(forced)::value) {
+      const uint32_t key = sr_key(St::kShifted ? sr_node_hash(ld.own) : 0u, i);
+      if (NT)
+        St::store_sr_nt(out0 + i * kTile, out, key);
+      else
+        St::store_sr(out0 + i * kTile, out, key);
+    } else {
+      if (NT)
+        St::store_nt(out0 + i * kTile, out);
+      else
+        St::store(out0 + i * kTile, out);
+    }
+  });
 }
 
 template <int Q, int S, bool LID, bool NT = false>
@@ -253,6 +280,211 @@ __device__ __forceinline__ void std_node_step(
   NodeLoad<Q, S> ld;
   std_gather<Q, S>(g, A, mask, cls, n, ld);
   std_collide<Q, S, LID, NT>(p, B, ld);
+}
+
+// ------------------------------------------------------ groups of NV nodes
+// Nodes per thread of the group step.
+constexpr int kGroup = 2;
+
+// Whether a lattice takes the group step: D2Q9 with 16-bit storage (its
+// one-node update is bound by instruction issue) and rows (along y) of
+// whole groups, when it streams from HBM (nt: both population copies
+// exceed the Infinity Cache).  Measured on MI355X, FP16: 4096^2 cavity
+// 110.5 vs 131.8 us per step, channel with squares 131.1 vs 148.2 us; at
+// 2048^2 and below (cache-resident) the group step is 0-17 % slower.  The
+// environment variable LBM_GROUP_STEP, read whenever a kernel is chosen,
+// overrides the size rule: 1 = wherever the shape allows, 0 = nowhere
+// (both steps give the same bytes).
+inline bool group_step_applies(int q, int storage, const Grid& g, bool nt) {
+  if (!(q == 9 && (storage == LBM_STORE_F16 || storage == LBM_STORE_BF16) && g.nz == 1 &&
+        g.ny % kGroup == 0))
+    return false;
+  const char* env = std::getenv("LBM_GROUP_STEP");
+  if (env && env[0] == '0') return false;
+  if (env && env[0] == '1') return true;
+  return nt;
+}
+
+// A thread updates the NV consecutive nodes r0..r0+NV-1 (r0 % NV == 0)
+// of one row along the fastest axis (y for D2Q9, z otherwise), so every
+// load and store of a direction moves NV values per lane (consecutive
+// elements of one tile run, aligned to NV elements), and the address
+// arithmetic is spent once per group instead of once per node.
+//
+// Gathering direction i reads the aligned group of the source row (the
+// row shifted by the other axes' components of -e_i) at r0..r0+NV-1.
+// Directions without a component along the row need exactly that group.
+// With +1 along the row the nodes need r0-1..r0+NV-2: NV-1 come from the
+// group, r0-1 is the last value of the previous group of the same source
+// row -- held by the thread of the previous group of this row, one lane
+// down -- fetched with a lane permute; -1 likewise takes r0+NV from the
+// next group's thread.  At the row ends the row wraps round (periodic;
+// with walls the link is bounced and the value replaced).  Where the
+// neighbouring group's thread sits in another wavefront (rows that do not
+// tile a wavefront) or loaded nothing (a group of solid nodes; the source
+// row need not be solid), the value is loaded instead.  Bounced links take
+// the node's own opposite population: one load of the group's own run per
+// direction that any of its nodes bounces, issued with the group loads.
+//
+// Callers: consecutive lanes of a wavefront take consecutive groups
+// (n0 = NV x (a multiple of 64 + lane)), n0 < N, and every thread whose
+// group lies in a row that any lane of the wavefront updates calls in
+// together (rows are whole, so only threads past the lattice end drop out).
+template <class T, int NV>
+struct VecOf {
+  typedef T V __attribute__((ext_vector_type(NV)));
+};
+
+// value v of lane `lane` of this wavefront (any lane index is harmless:
+// the permute uses it modulo 64); 16-bit values travel zero-extended
+template <class T>
+__device__ __forceinline__ T lane_value(T v, int lane) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(
+        T, __builtin_amdgcn_ds_bpermute(lane << 2, __builtin_bit_cast(int, v)));
+  } else {
+    const int w = (int)__builtin_bit_cast(uint16_t, v);
+    return __builtin_bit_cast(T, (uint16_t)__builtin_amdgcn_ds_bpermute(lane << 2, w));
+  }
+}
+
+template <int Q, int S, int NV, bool LID, bool NT>
+__device__ __forceinline__ void std_group_step(
+    const Grid& g, const typename Store<S>::T* __restrict__ A,
+    typename Store<S>::T* __restrict__ B, const uint32_t* __restrict__ mask,
+    const uint32_t* __restrict__ cls, const StdParams& p, uint32_t n0) {
+  using L = Lat<Q>;
+  using St = Store<S>;
+  using T = typename St::T;
+  using C = typename St::C;
+  using F = typename VecOf<T, NV>::V;
+  using U = typename VecOf<uint32_t, NV>::V;
+  constexpr int RA = Q == 9 ? 1 : 2;                  // the row axis
+  constexpr uint32_t kBits = (1u << (2 * NV)) - 1u;  // the group's class bits
+  const int lane = (int)(threadIdx.x & 63u);
+  uint32_t x, y, z;
+  if (Q == 9) {  // D2Q9 lattices have nz == 1
+    z = 0;
+    x = g.dy.div(n0);
+    y = n0 - __umul24(x, g.ny);
+  } else {
+    decode_node(g, n0, x, y, z);
+  }
+  const uint32_t plane = g.ny * g.nz;
+  // offsets from n0 to the source row for e = -1, 0, +1 along x and (3-D)
+  // y: multiples of the row length, so the source group is aligned too
+  const uint32_t ox[3] = {x + 1 == g.nx ? plane - g.n : plane, 0u,
+                          x == 0 ? g.n - plane : 0u - plane};
+  const uint32_t oy[3] = {y + 1 == g.ny ? g.nz - plane : g.nz, 0u,
+                          y == 0 ? plane - g.nz : 0u - g.nz};
+  auto source_row = [&](int i) -> uint32_t {
+    return Q == 9 ? n0 + ox[L::E[i][0] + 1] : n0 + ox[L::E[i][0] + 1] + oy[L::E[i][1] + 1];
+  };
+  const uint32_t r0 = Q == 9 ? y : z, nr = Q == 9 ? g.ny : g.nz;
+
+  // the group's 2-bit classes (2 NV bits of one class word), link masks
+  const uint32_t cb = (gload(cls + (n0 >> 4)) >> (2u * (n0 & 15u))) & kBits;
+  const bool all_solid = cb == (0xAAAAAAAAu & kBits);
+  U mv = {};
+  if (cb & 0x55555555u) mv = gload((const U*)(mask + n0));
+  bool solid[NV];
+  uint32_t m[NV];
+  uint32_t mor = 0;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    solid[k] = ((cb >> (2 * k)) & 3u) == kClassSolid;
+    m[k] = solid[k] ? 0u : mv[k];
+    mor |= m[k];
+  }
+
+  // the aligned source groups (nothing for a group of solid nodes) and the
+  // group's own opposite populations of bounced directions
+  F G[Q], O[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) G[i] = __builtin_nondeterministic_value(G[i]);
+  const uint32_t own = pidx<Q>(n0, 0);
+  if (!all_solid) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) G[i] = gload((const F*)elem(A, pidx<Q>(source_row(i), i)));
+  }
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    O[i] = __builtin_nondeterministic_value(O[i]);
+    if ((mor >> i) & 1u) O[i] = gload((const F*)elem(A, own + (uint32_t)L::INV[i] * kTile));
+  }
+
+  // neighbouring groups of the row: their lanes; the value is loaded
+  // instead where that lane is in another wavefront or loaded nothing;
+  // node offsets from the source group to r0-1 and r0+NV.  Every permute
+  // runs with all of the row's lanes active (a permute reads nothing from
+  // an inactive lane).
+  const int nq = (int)(nr / NV), rq = (int)(r0 / NV);
+  const int lane_lo = rq == 0 ? lane + nq - 1 : lane - 1;
+  const int lane_hi = rq + 1 == nq ? lane - nq + 1 : lane + 1;
+  const int skipped = all_solid ? 1 : 0;
+  const int lo_skipped = __builtin_amdgcn_ds_bpermute(lane_lo << 2, skipped);
+  const int hi_skipped = __builtin_amdgcn_ds_bpermute(lane_hi << 2, skipped);
+  const bool lo_far = !all_solid && ((unsigned)lane_lo >= 64u || lo_skipped != 0);
+  const bool hi_far = !all_solid && ((unsigned)lane_hi >= 64u || hi_skipped != 0);
+  const uint32_t d_lo = rq == 0 ? nr - 1u : 0u - 1u;
+  const uint32_t d_hi = rq + 1 == nq ? (uint32_t)NV - nr : (uint32_t)NV;
+
+  // V[i][k]: the stored value node r0+k pulls (or bounces) for direction i
+  T V[Q][NV];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    if (L::E[i][RA] == 0) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) V[i][k] = G[i][k];
+    } else if (L::E[i][RA] > 0) {  // source r - 1
+      T nb = lane_value(G[i][NV - 1], lane_lo);
+      // (a bounced link's value is replaced below: nothing to load)
+      if (lo_far && !((m[0] >> i) & 1u)) nb = gload(elem(A, pidx<Q>(source_row(i) + d_lo, i)));
+      V[i][0] = nb;
+#pragma unroll
+      for (int k = 1; k < NV; ++k) V[i][k] = G[i][k - 1];
+    } else {  // source r + 1
+      T nb = lane_value(G[i][0], lane_hi);
+      if (hi_far && !((m[NV - 1] >> i) & 1u))
+        nb = gload(elem(A, pidx<Q>(source_row(i) + d_hi, i)));
+#pragma unroll
+      for (int k = 0; k + 1 < NV; ++k) V[i][k] = G[i][k + 1];
+      V[i][NV - 1] = nb;
+    }
+  }
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    if ((mor >> i) & 1u) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) V[i][k] = ((m[k] >> i) & 1u) ? O[i][k] : V[i][k];
+    }
+  }
+
+  // collide the group's nodes (in the stored type again: rounded as the
+  // one-node step rounds), then store each direction's NV values
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    C gv[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) gv[i] = St::cvt(V[i][k]);
+    std_collide_node<Q, S, LID>(p, gv, m[k], solid[k], [&](auto forced, int i, C out) {
+      if constexpr (decltype(forced)::value)
+        V[i][k] = St::round_sr(out, sr_key(St::kShifted ? sr_node_hash(own + k) : 0u, i));
+      else
+        V[i][k] = St::round(out);
+    });
+  }
+  T* const out0 = B + own;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    F o;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) o[k] = V[i][k];
+    if (NT)
+      gstore_nt((F*)elem(out0, (uint32_t)i * kTile), o);
+    else
+      gstore((F*)elem(out0, (uint32_t)i * kTile), o);
+  }
 }
 
 }  // namespace lbm

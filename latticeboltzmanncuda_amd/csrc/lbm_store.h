@@ -55,9 +55,11 @@ struct Store<LBM_STORE_F32> {
   __device__ static C cvt(T v) { return v; }
   __device__ static T rest(int, double w) { return (T)w; }
   __device__ static C load(const T* p) { return gload(p); }
+  __device__ static T round(C v) { return v; }
+  // full-precision storage rounds to nearest
+  __device__ static T round_sr(C v, uint32_t) { return v; }
   __device__ static void store(T* p, C v) { gstore(p, v); }
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, v); }
-  // full-precision storage rounds to nearest
   __device__ static void store_sr(T* p, C v, uint32_t) { gstore(p, v); }
   __device__ static void store_sr_nt(T* p, C v, uint32_t) { gstore_nt(p, v); }
 };
@@ -77,6 +79,8 @@ struct Store<kStoreF32Shifted> {
   __device__ static C cvt(T v) { return v; }
   __device__ static T rest(int, double) { return 0.0f; }
   __device__ static C load(const T* p) { return gload(p); }
+  __device__ static T round(C v) { return v; }
+  __device__ static T round_sr(C v, uint32_t) { return v; }
   __device__ static void store(T* p, C v) { gstore(p, v); }
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, v); }
   __device__ static void store_sr(T* p, C v, uint32_t) { gstore(p, v); }
@@ -97,9 +101,11 @@ struct Store<LBM_STORE_F64> {
   __device__ static C cvt(T v) { return v; }
   __device__ static T rest(int, double w) { return w; }
   __device__ static C load(const T* p) { return gload(p); }
+  __device__ static T round(C v) { return v; }
+  // full-precision storage rounds to nearest
+  __device__ static T round_sr(C v, uint32_t) { return v; }
   __device__ static void store(T* p, C v) { gstore(p, v); }
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, v); }
-  // full-precision storage rounds to nearest
   __device__ static void store_sr(T* p, C v, uint32_t) { gstore(p, v); }
   __device__ static void store_sr_nt(T* p, C v, uint32_t) { gstore_nt(p, v); }
 };

@@ -1,6 +1,7 @@
 """Lattices large enough to take the streaming-store path (both population
 copies above 256 MB: non-temporal stores, alternating tile order) against
 the oracle, FP32 and FP16 storage."""
+import os
 import sys
 
 import numpy as np
@@ -39,15 +40,27 @@ def test_streaming_store_d3q19_f32_channel():
 
 
 def test_streaming_store_d2q9_f16_cavity():
-    # 2800 x 2800 D2Q9 FP16: 2 x 7.84 M x 9 x 2 B = 282 MB
+    # 2800 x 2800 D2Q9 FP16: 2 x 7.84 M x 9 x 2 B = 282 MB, which takes the
+    # group step (two nodes per thread); the one-node step gives the same
+    # bytes
     dims, q, steps = (2800, 2800, 1), 9, 6
     say(f"case: streaming stores D2Q9 F16 {dims} x {steps}")
     kw = dict(tau=0.6, lid_face=3, lid_u=(0.1, 0, 0))
-    with capi.Context(capi.make_config(*dims, lattice=q,
-                                       storage=capi.STORE_F16, **kw)) as c:
+    cfg = capi.make_config(*dims, lattice=q, storage=capi.STORE_F16, **kw)
+    with capi.Context(cfg) as c:
         c.init_equilibrium()
         c.step(steps)
         _, ug = c.moments()
+        fg = c.populations()
+    os.environ["LBM_GROUP_STEP"] = "0"
+    try:
+        with capi.Context(cfg) as c:
+            c.init_equilibrium()
+            c.step(steps)
+            f1 = c.populations()
+    finally:
+        del os.environ["LBM_GROUP_STEP"]
+    assert np.array_equal(fg, f1)
     mask = O.std_build_mask(q, dims, None, (0, 0, 0), 3)
     p = O.std_params(tau=0.6, lid_face=3, lid_u=(0.1, 0, 0))
     fo = O.std_run_f32(q, dims, O.std_init_eq(q, dims), mask, p, steps,

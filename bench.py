@@ -6,8 +6,9 @@ solid, Guo force 1e-6 along x, rho = 1, u = 0.
 
 A step is one fused pull collide+stream launch over the whole lattice with
 the populations already resident in HBM.  Prints one JSON line.  Besides the
-headline it carries three sections (skipped with --headline-only):
+headline it carries five sections (skipped with --headline-only):
 precision (Taylor-Green 256^3 in FP16/BF16/FP32 storage vs FP64),
+lattice2d (one D2Q9 FP16 lattice of 4096^2, group vs one-node step),
 batched (512 D2Q9 lattices, per-lattice launches vs the persistent batch
 kernel, FP32 and FP16), serving (Poisson requests through the
 continuous-batching server, batched vs per-lattice engine) and reference
@@ -48,11 +49,12 @@ def parse():
                     help="wall time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
-                    help="skip the precision, batched and serving sections")
+                    help="skip the extra sections")
     ap.add_argument("--sections",
-                    default="precision,batched,serving,reference",
+                    default="precision,lattice2d,batched,serving,reference",
                     help="comma list of the extra sections to run")
     ap.add_argument("--c3-steps", type=int, default=2000)
+    ap.add_argument("--l2d-steps", type=int, default=1000)
     ap.add_argument("--ref-steps", type=int, default=1000)
     ap.add_argument("--c4-steps", type=int, default=500)
     ap.add_argument("--c4-launch", default="",
@@ -121,6 +123,48 @@ def batched_section(steps, launch=None, chunk=100):
     out["f16"]["error_vs_f64"] = {
         k: (round(v, 8) if isinstance(v, float) else v)
         for k, v in err.items()}
+    return out
+
+
+def lattice2d_section(steps):
+    """One D2Q9 FP16 lattice of 4096^2 nodes (two population copies of
+    604 MB, streamed from HBM): lid-driven cavity and body-force channel
+    with 10 % squares, run by the group step (two nodes per thread, chosen
+    for lattices this size) and, for comparison, by the one-node step."""
+    from latticeboltzmanncuda_amd import capi, workloads
+    n = 4096
+    sq = workloads.random_squares(n, n, 0.1, np.random.default_rng(7))
+    out = {"workload": "D2Q9 FP16 4096^2 single lattice: cavity (lid 0.1, "
+                       "tau 0.6) and channel (force 1e-6, tau 0.8, 10% "
+                       "squares)", "steps": steps}
+    for name, kw, solid in (
+            ("cavity", dict(tau=0.6, lid_face=3, lid_u=(0.1, 0, 0)), None),
+            ("channel", dict(tau=0.8, periodic=(1, 0, 0), force=(1e-6, 0, 0)),
+             sq)):
+        res = {}
+        for step, env in (("group", None), ("one_node", "0")):
+            if env is None:
+                os.environ.pop("LBM_GROUP_STEP", None)
+            else:
+                os.environ["LBM_GROUP_STEP"] = env
+            try:
+                cfg = capi.make_config(n, n, 1, lattice=capi.D2Q9,
+                                       storage=capi.STORE_F16, **kw)
+                with capi.Context(cfg) as c:
+                    c.upload_solid(solid)
+                    c.init_equilibrium()
+                    c.step(20)
+                    c.synchronize()
+                    us = c.time_steps(steps) / steps * 1e3
+            finally:
+                os.environ.pop("LBM_GROUP_STEP", None)
+            res[step] = {"us_per_step": round(us, 2),
+                         "mlups": round(n * n / us, 1),
+                         "hbm_frac_36B": round(n * n * 36 / (us * 1e-6) / 8e12,
+                                               4)}
+        res["speedup"] = round(res["one_node"]["us_per_step"]
+                               / res["group"]["us_per_step"], 3)
+        out[name] = res
     return out
 
 
@@ -397,6 +441,9 @@ def main():
     if "precision" in sections:
         log("precision")
         out["precision"] = precision_section(args.c3_steps)
+    if "lattice2d" in sections:
+        log("lattice2d")
+        out["lattice2d"] = lattice2d_section(args.l2d_steps)
     if "batched" in sections:
         log("batched")
         launch = (tuple(int(v) for v in args.c4_launch.split(","))

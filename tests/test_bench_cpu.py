@@ -21,7 +21,8 @@ def load_bench():
 
 def test_bench_defines_every_section():
     b = load_bench()
-    for name in ("parse", "precision_section", "batched_section",
+    for name in ("parse", "precision_section", "lattice2d_section",
+                 "batched_section",
                  "serving_section", "reference_mode_section",
                  "cpu_baseline", "measured_traffic", "main"):
         assert callable(getattr(b, name)), name

@@ -12,6 +12,8 @@
 // use ordinary stores and a fixed order (both measured fastest on MI355X).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "lbm_internal.h"
 #include "lbm_lattice.h"
 #include "lbm_layout.h"
@@ -43,7 +45,7 @@ __global__ void __launch_bounds__(kBlock)
   std_node_step<Q, S, LID, NT>(g, A, B, mask, cls, p, n);
 }
 
-// Groups of kGroup nodes per thread (std_group_step).
+// Groups of NV nodes per thread (std_group_step).
 template <int Q, int S, int NV, bool LID, bool NT>
 __global__ void __launch_bounds__(kBlock)
     k_std_stepv(Grid g, const typename Store<S>::T* __restrict__ A,
@@ -307,14 +309,14 @@ int launch_std_step(int q, int storage, const Grid& g, const void* A, void* B,
   // between steps is fastest; without them, a fixed order is
   const int r = nt ? rev : 0;
   const dim3 grid(nb), block(kBlock);
-  if (group_step_applies(q, storage, g, nt)) {
-    const dim3 gridv((g.n / kGroup + kBlock - 1) / kBlock);
-    auto go = [&](auto tag) {
+  if (const int width = group_step_width(q, storage, g, nt)) {
+    const dim3 gridv((g.n / (uint32_t)width + kBlock - 1) / kBlock);
+    auto go = [&](auto tag, auto nv) {
       using Tag = decltype(tag);
       using T = typename Store<Tag::S>::T;
+      constexpr int V = decltype(nv)::value;
       const T* a = (const T*)A;
       T* b = (T*)B;
-      constexpr int V = kGroup;
       if (lid && nt)
         hipLaunchKernelGGL((k_std_stepv<9, Tag::S, V, true, true>), gridv, block, 0, s, g, a, b,
                            mask, cls, p, r);
@@ -328,10 +330,12 @@ int launch_std_step(int q, int storage, const Grid& g, const void* A, void* B,
         hipLaunchKernelGGL((k_std_stepv<9, Tag::S, V, false, false>), gridv, block, 0, s, g, a,
                            b, mask, cls, p, r);
     };
+    using Two = std::integral_constant<int, 2>;
+    using Four = std::integral_constant<int, 4>;
     if (storage == LBM_STORE_F16)
-      go(QS<9, LBM_STORE_F16>{});
+      width == 4 ? go(QS<9, LBM_STORE_F16>{}, Four{}) : go(QS<9, LBM_STORE_F16>{}, Two{});
     else
-      go(QS<9, LBM_STORE_BF16>{});
+      width == 4 ? go(QS<9, LBM_STORE_BF16>{}, Four{}) : go(QS<9, LBM_STORE_BF16>{}, Two{});
     LBM_HIP(hipGetLastError());
     return LBM_OK;
   }

@@ -29,9 +29,9 @@
 // constants come precomputed (StepCoef), addresses need no 32-bit integer
 // multiplications, the body-force terms run only for lattices that have
 // one, and solid lanes get no values.  Large D2Q9 lattices with 16-bit
-// storage take the group step (std_group_step, below): two nodes per thread
-// share their loads, stores and address arithmetic, with the same per-node
-// arithmetic, so both steps give the same bytes.
+// storage take the group step (std_group_step, below): two or four nodes
+// per thread share their loads, stores and address arithmetic, with the
+// same per-node arithmetic, so every width gives the same bytes.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -283,26 +283,25 @@ __device__ __forceinline__ void std_node_step(
 }
 
 // ------------------------------------------------------ groups of NV nodes
-// Nodes per thread of the group step.
-constexpr int kGroup = 2;
-
-// Whether a lattice takes the group step: D2Q9 with 16-bit storage (its
-// one-node update is bound by instruction issue) and rows (along y) of
-// whole groups, when it streams from HBM (nt: both population copies
-// exceed the Infinity Cache).  Measured on MI355X, FP16: 4096^2 cavity
-// 110.5 vs 131.8 us per step, channel with squares 131.1 vs 148.2 us; at
-// 2048^2 and below (cache-resident) the group step is 0-17 % slower.  The
-// environment variable LBM_GROUP_STEP, read whenever a kernel is chosen,
-// overrides the size rule: 1 = wherever the shape allows, 0 = nowhere
-// (both steps give the same bytes).
-inline bool group_step_applies(int q, int storage, const Grid& g, bool nt) {
-  if (!(q == 9 && (storage == LBM_STORE_F16 || storage == LBM_STORE_BF16) && g.nz == 1 &&
-        g.ny % kGroup == 0))
-    return false;
+// Nodes per thread of the group step for a lattice: 4 (rows along y of a
+// multiple of 4 nodes), 2 (even rows), or 0 = the one-node step.  It is
+// taken by D2Q9 lattices with 16-bit storage (their one-node update is
+// bound by instruction issue) that stream from HBM (nt: both population
+// copies exceed the Infinity Cache).  Measured on MI355X, FP16, per step:
+// 4096^2 cavity 131.7 us one node per thread, 110.6 us two, 104.8 us four
+// (eight: 110.0); at 2048^2 and below (cache-resident) the group step is
+// 0-17 % slower than one node per thread.  The environment variable
+// LBM_GROUP_STEP, read whenever a kernel is chosen, overrides the size
+// rule: 1 = wherever the row length allows, 0 = nowhere (every width gives
+// the same bytes).
+inline int group_step_width(int q, int storage, const Grid& g, bool nt) {
+  if (!(q == 9 && (storage == LBM_STORE_F16 || storage == LBM_STORE_BF16) && g.nz == 1))
+    return 0;
+  const int width = g.ny % 4 == 0 ? 4 : (g.ny % 2 == 0 ? 2 : 0);
   const char* env = std::getenv("LBM_GROUP_STEP");
-  if (env && env[0] == '0') return false;
-  if (env && env[0] == '1') return true;
-  return nt;
+  if (env && env[0] == '0') return 0;
+  if (env && env[0] == '1') return width;
+  return nt ? width : 0;
 }
 
 // A thread updates the NV consecutive nodes r0..r0+NV-1 (r0 % NV == 0)

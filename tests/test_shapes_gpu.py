@@ -1,8 +1,9 @@
 """Standard-mode steps on unusual shapes -- FP32 D3Q19 / D3Q27, and D2Q9
-with 16-bit storage run by the group step (two nodes per thread where ny
-is even; LBM_GROUP_STEP=1 selects it below the size where it is chosen by
-itself) -- against the one-node step (LBM_GROUP_STEP=0) and the one-lattice
-batch (same bytes), and against the oracle.  Shapes cover thin and long z-rows (nz = 4 to 260, tiles of 64
+with 16-bit storage run by the group step (four nodes per thread where ny
+is a multiple of 4, two where it is even; LBM_GROUP_STEP=1 selects it below
+the size where it is chosen by itself) -- against the one-node step
+(LBM_GROUP_STEP=0) and the one-lattice batch (same bytes), and against the
+oracle.  Shapes cover thin and long z-rows (nz = 4 to 260, tiles of 64
 nodes spanning several rows or a part of one), periodic and walled z, a
 moving wall, a body force, and runs of solid nodes beside fluid rows
 (solid lanes load nothing; their fluid neighbours' bounced links must
@@ -58,6 +59,10 @@ CASES = [
     (9, (17, 258, 1), (1, 1, 0), 0, False, False, F16),
     (9, (24, 6, 1), (0, 0, 0), 1, True, True, F16),
     (9, (30, 31, 1), (1, 0, 0), 0, False, True, F16),  # odd ny
+    # ny % 4 == 0: four nodes per thread
+    (9, (16, 260, 1), (1, 1, 0), 1, False, True, F16),
+    (9, (12, 8, 1), (0, 0, 0), None, True, True, F16),
+    (9, (20, 132, 1), (0, 0, 0), 0, True, False, BF16),
 ]
 
 

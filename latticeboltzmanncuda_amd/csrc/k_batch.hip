@@ -1,5 +1,5 @@
 // Batched kernels: per step, one persistent launch per kernel class
-// (lattice type x moving wall) updates every resident lattice of that class
+// (lattice type) updates every resident lattice of that class
 // (a batch mixes D2Q9 / D3Q19 / D3Q27 with one storage type), and a
 // residual kernel measures |u - u_prev| / |u| per lattice.
 #include <hip/hip_runtime.h>
@@ -18,7 +18,7 @@ namespace {
 // uniform across the workgroup.
 constexpr int kMaxBatchBlock = 512;
 
-template <int Q, int S, bool LID, bool NT>
+template <int Q, int S, bool NT>
 __global__ void __launch_bounds__(kMaxBatchBlock)
     k_batch_step(const BatchLattice* __restrict__ lats,
                  const BatchTile* __restrict__ tiles,
@@ -31,8 +31,14 @@ __global__ void __launch_bounds__(kMaxBatchBlock)
     const uint32_t n = tile.first + threadIdx.x;
     if (n < L.g.n) {
       const int cur = (L.off + parity) & 1;
-      std_node_step<Q, S, LID, NT>(L.g, (const T*)L.f[cur], (T*)L.f[cur ^ 1], L.mask, L.cls,
-                                   L.p, n);
+      // the moving-wall code only for lattices that have one (uniform per
+      // tile)
+      if (L.p.lid_axis >= 0)
+        std_node_step<Q, S, true, NT>(L.g, (const T*)L.f[cur], (T*)L.f[cur ^ 1], L.mask, L.cls,
+                                      L.p, n);
+      else
+        std_node_step<Q, S, false, NT>(L.g, (const T*)L.f[cur], (T*)L.f[cur ^ 1], L.mask, L.cls,
+                                       L.p, n);
     }
   }
 }
@@ -119,17 +125,13 @@ __global__ void __launch_bounds__(kResidualChunk)
 }
 
 template <int Q, int S>
-int launch_step_q(bool lid, bool nt, int block, int grid, const BatchLattice* lats,
-                  const BatchTile* tiles, const uint32_t* ntiles, int parity, hipStream_t s) {
+int launch_step_q(bool nt, int block, int grid, const BatchLattice* lats, const BatchTile* tiles,
+                  const uint32_t* ntiles, int parity, hipStream_t s) {
   const dim3 g((unsigned)grid), b((unsigned)block);
-  if (lid && nt)
-    hipLaunchKernelGGL((k_batch_step<Q, S, true, true>), g, b, 0, s, lats, tiles, ntiles, parity);
-  else if (lid)
-    hipLaunchKernelGGL((k_batch_step<Q, S, true, false>), g, b, 0, s, lats, tiles, ntiles, parity);
-  else if (nt)
-    hipLaunchKernelGGL((k_batch_step<Q, S, false, true>), g, b, 0, s, lats, tiles, ntiles, parity);
+  if (nt)
+    hipLaunchKernelGGL((k_batch_step<Q, S, true>), g, b, 0, s, lats, tiles, ntiles, parity);
   else
-    hipLaunchKernelGGL((k_batch_step<Q, S, false, false>), g, b, 0, s, lats, tiles, ntiles, parity);
+    hipLaunchKernelGGL((k_batch_step<Q, S, false>), g, b, 0, s, lats, tiles, ntiles, parity);
   LBM_HIP(hipGetLastError());
   return LBM_OK;
 }
@@ -143,11 +145,10 @@ int launch_batch_step(int storage, int cls, bool nt, int block, int grid,
   if (block != 64 && block != 128 && block != 256 && block != kMaxBatchBlock)
     return fail(LBM_ERR_INVALID, "batch block size must be 64, 128, 256 or 512");
   if (cls < 0 || cls >= kBatchClasses) return fail(LBM_ERR_INVALID, "bad kernel class");
-  const bool lid = class_lid(cls);
   int rc = LBM_OK;
   const int drc = dispatch_qs(class_q(cls), storage, [&](auto tag) {
     using Tag = decltype(tag);
-    rc = launch_step_q<Tag::Q, Tag::S>(lid, nt, block, grid, lats, tiles, ntiles, parity, s);
+    rc = launch_step_q<Tag::Q, Tag::S>(nt, block, grid, lats, tiles, ntiles, parity, s);
   });
   return drc ? drc : rc;
 }

@@ -96,7 +96,7 @@ int sync_tables(lbm_batch* b) {
     d.res_first = (uint32_t)chunks.size();
     pop_bytes += 2 * population_elems(c->q, c->grid.n) * storage_bytes(b->storage);
     const uint32_t idx = (uint32_t)b->h_lats.size();
-    const int cls = batch_class(c->q, d.p.lid_axis >= 0);
+    const int cls = batch_class(c->q);
     for (uint32_t first = 0; first < c->grid.n; first += (uint32_t)b->block)
       tiles[cls].push_back({idx, first});
     uint32_t nres = 0;
@@ -336,7 +336,6 @@ int lbm_batch_add(lbm_batch* b, const lbm_config* cfg, const uint8_t* solid,
   lbm_ctx* c = nullptr;
   int rc = lbm_create(cfg, b->stream, &c);
   if (rc) return rc;
-  c->batch_member = true;
   if (solid && (rc = lbm_upload_solid(c, solid))) {
     lbm_destroy(c);
     return rc;

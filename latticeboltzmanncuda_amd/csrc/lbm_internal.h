@@ -169,7 +169,6 @@ struct lbm_ctx {
   uint32_t* cls = nullptr;   // standard mode: 2-bit node classes
   uint64_t steps_done = 0;   // parity picks the tile order of the next step
   uint64_t params_version = 0;  // bumped when tau or the body force change
-  bool batch_member = false;    // owned by a batch (its kernel class is fixed)
   uint8_t* flags = nullptr;  // reference mode
   std::vector<uint8_t> host_flags;
 

@@ -163,9 +163,6 @@ int lbm_snapshot_load(lbm_ctx* c, const char* path) {
   // the snapshot's parameters replace the context's and pass the checks a
   // new context's would (the geometry tables come with the snapshot)
   if (int e = check_config(&s)) return e;
-  // a batch member's kernel class is fixed by whether it has a moving wall
-  if (c->batch_member && s.lid_face != c->cfg.lid_face)
-    return fail(LBM_ERR_INVALID, "a batch member's moving wall cannot change");
   const uint64_t N = c->grid.n;
   const bool standard = c->cfg.mode == LBM_MODE_STANDARD;
   if (h.pop_bytes != population_elems(c->q, N) * storage_bytes(c->cfg.storage) ||

@@ -6,7 +6,9 @@
 // One thread per node over a flat 1-D grid (coalesced along n, whatever the
 // lattice shape); (x, y, z) come from two magic-number divisions.
 //
-// The per-node update is std_node_step (lbm_step_node.h).  Lattices whose
+// The per-node update is std_node_step (lbm_step_node.h); large D2Q9
+// lattices with 16-bit storage take std_group_step instead, 2 or 4 nodes
+// per thread (k_std_stepv, chosen by group_step_width).  Lattices whose
 // two population copies exceed the Infinity Cache use non-temporal stores
 // and walk the tiles in opposite orders on consecutive steps; smaller ones
 // use ordinary stores and a fixed order (both measured fastest on MI355X).

@@ -202,8 +202,12 @@ int lbm_set_force(lbm_ctx* ctx, const float in[3]);
 int lbm_get_force(lbm_ctx* ctx, float out[3]);
 
 /* ------------------------------------------------------------ stepping */
-/* n fused steps.  Standard mode: BGK pull collide+stream.  Reference mode:
- * Lattice::simulateLattice (Lattice.cpp:222-226), stream then collide. */
+/* n fused steps.  Standard mode: BGK pull collide+stream (D2Q9 lattices
+ * with 16-bit storage larger than the Infinity Cache are updated 4 or 2
+ * nodes per thread; the environment variable LBM_GROUP_STEP=0 / 1 turns
+ * that off / on for any size -- the results are the same bytes).
+ * Reference mode: Lattice::simulateLattice (Lattice.cpp:222-226), stream
+ * then collide. */
 int lbm_step(lbm_ctx* ctx, int n);
 /* n steps of each of count standard-mode lattices, one lbm_step launch
  * per lattice per step (the unbatched way to step many lattices; no

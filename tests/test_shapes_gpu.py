@@ -63,6 +63,7 @@ CASES = [
     (9, (16, 260, 1), (1, 1, 0), 1, False, True, F16),
     (9, (12, 8, 1), (0, 0, 0), None, True, True, F16),
     (9, (20, 132, 1), (0, 0, 0), 0, True, False, BF16),
+    (9, (8, 64, 1), (1, 0, 0), 1, False, True, BF16),
 ]
 
 

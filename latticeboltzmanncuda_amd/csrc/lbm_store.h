@@ -32,12 +32,14 @@
 namespace lbm {
 
 // Random bits for stochastic rounding of the float with bit pattern b:
-// key = node hash + direction constant (sr_key).  One 24-bit multiply mixes
-// the low 24 bits of the value with the key; callers take the top bits of
-// the product (taking low ones would let the compiler drop the 24-bit
-// operand mask and fall back to the quarter-rate 32-bit multiply).
+// key = node hash + direction constant (sr_key).  One 24-bit multiply-add,
+// the low 24 bits of the value times an odd constant plus the key: for a
+// given value the key, uniform over nodes, makes the bits uniform, and a
+// node's bits change from step to step with its values.  Callers take the
+// top bits (taking low ones would let the compiler drop the 24-bit operand
+// mask and fall back to the quarter-rate 32-bit multiply).
 __device__ __forceinline__ uint32_t sr_bits(uint32_t b, uint32_t key) {
-  return __umul24(b ^ key, 0x2C1B3Du);
+  return __umul24(b, 0x2C1B3Du) + key;
 }
 __device__ __forceinline__ uint32_t sr_node_hash(uint32_t own) { return own * 0x9E3779B1u; }
 __device__ __forceinline__ uint32_t sr_key(uint32_t node_hash, int i) {
@@ -120,14 +122,16 @@ struct Store<LBM_STORE_F16> {
   __device__ static T rest(int, double) { return (T)0; }
   __device__ static C load(const T* p) { return cvt(gload(p)); }
   __device__ static T round(C v) { return (_Float16)(v * kScale); }
-  // the 13 bits FP32 -> FP16 drops get a random addend, then are cut; a
-  // result in the FP16 subnormal range (below 1.5e-8 unscaled) is rounded
-  // to nearest by the conversion
+  // the 13 bits FP32 -> FP16 drops get a random addend, then are cut (on
+  // the unscaled value: scaling by 2^12 keeps the mantissa, and the exact
+  // product is converted in the same instruction); a result in the FP16
+  // subnormal range (below 1.5e-8 unscaled) is rounded to nearest by the
+  // conversion
   __device__ static T round_sr(C v, uint32_t key) {
-    const uint32_t b = __float_as_uint(v * kScale);
+    const uint32_t b = __float_as_uint(v);
     const uint32_t r = sr_bits(b, key) >> 19;  // 13 bits
     // saturating add: a NaN's pattern cannot wrap round to a number
-    return (_Float16)__uint_as_float(__builtin_elementwise_add_sat(b, r) & 0xFFFFE000u);
+    return (_Float16)(__uint_as_float(__builtin_elementwise_add_sat(b, r) & 0xFFFFE000u) * kScale);
   }
   __device__ static void store(T* p, C v) { gstore(p, round(v)); }
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, round(v)); }

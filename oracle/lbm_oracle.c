@@ -866,22 +866,23 @@ static uint32_t orc_sat_add(uint32_t a, uint32_t b) {
 
 static float orc_store16(int storage, float v, int sr, uint32_t key) {
   if (storage == 2) {
-    float y = v * 4096.0f;
     if (sr) {
+      /* the 13 bits the FP16 conversion drops, cut on the unscaled value
+       * (scaling by 2^12 keeps the mantissa): the top 13 bits of the low
+       * 32 bits of (24-bit value bits) x constant + key */
       uint32_t b;
-      memcpy(&b, &y, 4);
-      /* low 32 bits of the 24-bit product; its top 13 bits */
-      const uint32_t r = (((b ^ key) & 0xffffffu) * 0x2C1B3Du) >> 19;
+      memcpy(&b, &v, 4);
+      const uint32_t r = ((b & 0xffffffu) * 0x2C1B3Du + key) >> 19;
       b = orc_sat_add(b, r) & 0xffffe000u;
-      memcpy(&y, &b, 4);
+      memcpy(&v, &b, 4);
     }
-    return orc_round_fp16(y) / 4096.0f;
+    return orc_round_fp16(v * 4096.0f) / 4096.0f;
   }
   if (storage == 3) {
     if (!sr) return orc_round_bf16(v);
     uint32_t b;
     memcpy(&b, &v, 4);
-    const uint32_t r = (((b ^ key) & 0xffffffu) * 0x2C1B3Du) >> 16;
+    const uint32_t r = ((b & 0xffffffu) * 0x2C1B3Du + key) >> 16;
     b = orc_sat_add(b, r) & 0xffff0000u;
     float o;
     memcpy(&o, &b, 4);

@@ -21,9 +21,11 @@ constexpr int kMaxBatchBlock = 512;
 // Eight waves per SIMD: the compiler then keeps the kernel within 80
 // scalar registers (spilling a few to vector-register lanes), which on
 // MI355X is what lets eight waves stay resident -- the D3Q19 kernels used
-// 105-106 and ran a D3Q19 FP16 batch 3 % slower.
+// 105-106 and ran a D3Q19 FP16 batch 3 % slower.  Not for FP64 storage,
+// whose D3Q19 / D3Q27 kernels would spill vector registers to memory.
 template <int Q, int S, bool NT>
-__global__ void __launch_bounds__(kMaxBatchBlock) __attribute__((amdgpu_waves_per_eu(8)))
+__global__ void __launch_bounds__(kMaxBatchBlock)
+    __attribute__((amdgpu_waves_per_eu(S == LBM_STORE_F64 ? 1 : 8)))
     k_batch_step(const BatchLattice* __restrict__ lats,
                  const BatchTile* __restrict__ tiles,
                  const uint32_t* __restrict__ ntiles, int parity) {

@@ -891,6 +891,10 @@ static float orc_store16(int storage, float v, int sr, uint32_t key) {
   return v;
 }
 
+float orc_store16_sr(int storage, float v, uint32_t key) {
+  return orc_store16(storage, v, 1, key);
+}
+
 /* One pull step of the standard model for one node, float compute.
  *   f_i = A_i(x - e_i)                  open link (periodic wrap)
  *   f_i = A_inv(i)(x) + 6 w_i rho_w e_i.u_w   bounced link (wall term only

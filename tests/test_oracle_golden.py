@@ -549,3 +549,45 @@ def test_std_fp16_forced_channel_does_not_stall():
     ob = O.std_run_f32(q, dims, f, mask, p, 2000, O.STORE_BF16_SHIFTED)
     _, ub = O.std_moments(q, dims, ob, mask, (5e-8, 0, 0))
     assert np.sqrt(((ub - u64) ** 2).sum() / (u64 ** 2).sum()) < 2e-2
+
+
+def _f16_neighbours(y):
+    # the two binary16 values around y (y itself twice when representable)
+    h = np.float16(y)
+    if float(h) == y:
+        return float(h), float(h)
+    lo = h if float(h) < y else np.nextafter(h, np.float16(-np.inf))
+    hi = np.nextafter(lo, np.float16(np.inf))
+    return float(lo), float(hi)
+
+
+def _bf16_neighbours(v):
+    b = int(np.float32(v).view(np.uint32))
+    t = np.uint32(b & 0xFFFF0000).view(np.float32)
+    u = np.uint32((b & 0xFFFF0000) + 0x10000).view(np.float32)
+    return tuple(sorted((float(t), float(u))))
+
+
+@pytest.mark.parametrize("storage", [O.STORE_F16_SHIFTED, O.STORE_BF16_SHIFTED])
+def test_stochastic_rounding_is_unbiased(storage):
+    # The 16-bit store of a forced lattice: every result is one of the two
+    # neighbouring 16-bit values, and over node keys (uniform, as the node
+    # hash makes them) its mean is the stored value itself -- rounding
+    # to nearest would be biased by up to half a unit for the same values.
+    # A wrong shift, mask or dropped key fails one of the two.
+    rng = np.random.default_rng(11)
+    keys = rng.integers(0, 2 ** 32, 3000, dtype=np.uint64)
+    scale = 4096.0 if storage == O.STORE_F16_SHIFTED else 1.0
+    for v in (rng.choice([-1, 1], 24) * 10 ** rng.uniform(-6, -1, 24)).astype(np.float32):
+        v = float(v)
+        if storage == O.STORE_F16_SHIFTED:
+            lo, hi = _f16_neighbours(np.float32(v * scale))
+        else:
+            lo, hi = _bf16_neighbours(v)
+        if lo == hi:
+            continue
+        got = np.array([O.store16_sr(storage, v, int(k)) * scale for k in keys])
+        assert np.all((got == lo) | (got == hi)), (v, set(got) - {lo, hi})
+        p = (v * scale - lo) / (hi - lo)  # exact rounding-up probability
+        sd = (hi - lo) * np.sqrt(p * (1 - p) / len(keys))
+        assert abs(got.mean() - v * scale) <= 4.5 * sd + 1e-12 * abs(v * scale), (v, got.mean(), sd)

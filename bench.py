@@ -64,7 +64,7 @@ def parse():
                     help="steps per hipGraph replay of the batch (0: plain "
                          "launches, e.g. under a kernel tracer)")
     ap.add_argument("--c5-requests", type=int, default=64)
-    ap.add_argument("--c5-lambda", default="100",
+    ap.add_argument("--c5-lambda", default="10,200",
                     help="comma list of arrival rates (requests/s)")
     ap.add_argument("--c5-engines", default="batched,per_lattice",
                     help="batched, per_lattice, batched_capped (admission "
@@ -98,10 +98,17 @@ def batched_section(steps, launch=None, chunk=100):
     from latticeboltzmanncuda_amd import capi, workloads
     out = {"workload": "512 D2Q9 lattices (seed 4), sides log-uniform "
                        "32-512, cavity/channel, Re 100-1000, tau >= 0.51, "
-                       "10% squares", "steps": steps, "chunk": chunk}
-    specs32, clamped = workloads.c4_specs(512, storage=capi.STORE_F32)
+                       "10% squares; admitted if cell Reynolds number "
+                       f"U/nu <= {workloads.C4_MAX_CELL_RE:g}",
+           "steps": steps, "chunk": chunk}
+    drawn, clamped = workloads.c4_specs(512, storage=capi.STORE_F32)
+    specs32, rejected = workloads.c4_gate(drawn)
     out["tau_clamped"] = clamped
+    out["max_cell_re"] = workloads.C4_MAX_CELL_RE
+    out["lattices"] = len(specs32)
+    out["rejected_unstable"] = rejected
     out["nodes"] = int(sum(c.nx * c.ny for c, _, _ in specs32))
+    out["nodes_drawn"] = int(sum(c.nx * c.ny for c, _, _ in drawn))
     for name, st, bpl in (("f32", capi.STORE_F32, 72),
                           ("f16", capi.STORE_F16, 36)):
         specs = [(workloads.with_storage(c, st), s, i)
@@ -191,8 +198,15 @@ def serving_section(nreq, lams, max_steps, engines, node_cap):
                                  engine=eng[name][0], progress_s=30.0,
                                  max_resident_nodes=eng[name][1])
             conv = sum(1 for r in res if r["status"] == 0)
+            gbs = (st["population_bytes"] / st["wall_s"] / 1e9
+                   if st["wall_s"] > 0 else 0.0)
             row[name] = {
                 "mlups": round(st["mlups"], 1),
+                # algorithmic population bytes (2 Q s per node and step,
+                # Q 9 or 19, FP16) over the wall time, against 8 TB/s
+                "population_gbs": round(gbs, 1),
+                "hbm_frac": round(gbs / 8000.0, 4),
+                "unstable": st["unstable"], "diverged": st["diverged"],
                 "p50_latency_s": round(st["p50_latency_s"], 4),
                 "p99_latency_s": round(st["p99_latency_s"], 4),
                 "mean_latency_s": round(st["mean_latency_s"], 4),

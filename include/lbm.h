@@ -43,7 +43,8 @@ enum lbm_status {
   LBM_ERR_STATE = 4,       /* call not valid in the context's mode     */
   LBM_ERR_UNSUPPORTED = 5, /* combination not built                    */
   LBM_ERR_BUDGET = 6,      /* HBM budget of a batch would be exceeded  */
-  LBM_ERR_NOTFOUND = 7     /* unknown handle / slot                    */
+  LBM_ERR_NOTFOUND = 7,    /* unknown handle / slot                    */
+  LBM_ERR_UNSTABLE = 8     /* lattice beyond the admission stability limit */
 };
 
 /* ------------------------------------------------------------ enums */
@@ -85,6 +86,11 @@ enum lbm_flow_kind {
  * SOLID = node is solid and is not updated. */
 #define LBM_MASK_SOLID (1u << 31)
 #define LBM_MASK_LID (1u << 30)
+
+/* A lattice whose speed |u| exceeds this anywhere (lattice units; the
+ * speed of sound is 1/sqrt(3) ~ 0.577, and stable BGK flows stay far
+ * below it) counts as diverged, as does a non-finite velocity. */
+#define LBM_DIVERGED_SPEED 0.5
 
 /* lid_face: -1 = none, else 2*axis + side (side 0 = min face, 1 = max) */
 typedef struct lbm_config {
@@ -255,6 +261,19 @@ int64_t lbm_batch_estimate_bytes(const lbm_config* cfg, int track_residual);
 int lbm_batch_add(lbm_batch* b, const lbm_config* cfg, const uint8_t* solid,
                   int track_residual, int32_t* slot);
 int lbm_batch_remove(lbm_batch* b, int32_t slot);
+/* BGK cell Reynolds number U / nu of a standard-mode configuration, nu =
+ * (tau - 1/2) / 3 in lattice units, U the larger of the moving wall's
+ * speed |u_w| and, with a body force F, the centre-line speed of a
+ * Poiseuille channel across the narrowest walled extent W, |F| W^2 /
+ * (8 nu) (W = the largest extent when no axis has walls).  BGK flows
+ * driven at U ~ 0.1 turn unstable as U / nu approaches ~17 (lid-driven
+ * cavities, measured); a negative value means an invalid cfg.  No
+ * counterpart in the reference, which has no stability check. */
+double lbm_cell_reynolds(const lbm_config* cfg);
+/* Admission limit of a batch: lbm_batch_add refuses a lattice whose
+ * lbm_cell_reynolds exceeds max_cell_re with LBM_ERR_UNSTABLE; 0 (the
+ * default) admits every lattice. */
+int lbm_batch_set_stability_limit(lbm_batch* b, double max_cell_re);
 /* The member's context, for the per-lattice calls (init, populations,
  * moments, conservation).  Do not lbm_step or lbm_destroy a member. */
 int lbm_batch_member(lbm_batch* b, int32_t slot, lbm_ctx** ctx);
@@ -279,7 +298,9 @@ int lbm_batch_synchronize(lbm_batch* b);
 int lbm_batch_time_steps(lbm_batch* b, int n, int chunk, float* ms);
 /* out[slot] = |u - u_prev|_2 / |u|_2 over fluid nodes of every tracked
  * member since its previous call (the snapshot is then replaced); -1 for
- * empty or untracked slots.  nslots >= the number of slots. */
+ * empty or untracked slots; NaN for a member that diverged: a non-finite
+ * velocity at any fluid node, or a speed above LBM_DIVERGED_SPEED.
+ * nslots >= the number of slots. */
 int lbm_batch_residuals(lbm_batch* b, double* out, int32_t nslots);
 
 /* ------------------------------------------------------------ serving */
@@ -292,7 +313,10 @@ int lbm_batch_residuals(lbm_batch* b, double* out, int32_t nslots);
  * or after max_steps (rounded up to whole chunks).  The final velocity (3
  * planes of N floats) is copied to out_u (or a scratch buffer) before
  * t_done_s is taken.  status: 0 converged, 1 step limit, 2 rejected (does
- * not fit the budget), 3 diverged. */
+ * not fit the budget), 3 diverged (retired at the chunk where its velocity
+ * turned non-finite or faster than LBM_DIVERGED_SPEED; the other
+ * requests are unaffected), 4 rejected as unstable (cell Reynolds number
+ * above the server's max_cell_re; not run). */
 typedef struct lbm_request {
   lbm_config cfg;       /* standard mode, storage = the server's */
   const uint8_t* solid; /* one byte per node, or NULL */
@@ -317,6 +341,8 @@ typedef struct lbm_serve_params {
   int64_t max_resident_nodes; /* 0 = no limit; otherwise admit only while
                                  the resident lattices hold fewer nodes
                                  (a lattice alone is always admitted) */
+  double max_cell_re;   /* > 0: requests whose lbm_cell_reynolds exceeds
+                           it are rejected with status 4; 0 = no limit */
 } lbm_serve_params;
 
 /* how a server advances its resident lattices by one chunk */
@@ -332,6 +358,11 @@ typedef struct lbm_serve_stats {
   double p50_latency_s, p99_latency_s, mean_latency_s;
   int32_t completed, rejected, max_resident;
   int64_t peak_bytes;
+  int32_t unstable;         /* rejected by max_cell_re (not in rejected) */
+  int32_t diverged;         /* retired with status 3 (in completed) */
+  double population_bytes;  /* algorithmic HBM bytes of the steps: sum
+                               over chunks and resident lattices of nodes
+                               x 2 Q x storage bytes x chunk */
 } lbm_serve_stats;
 
 int lbm_serve_run(const lbm_serve_params* params, lbm_request* requests,

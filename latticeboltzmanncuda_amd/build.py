@@ -45,7 +45,8 @@ HIP_FLAGS = [
 
 HIP_SOURCES = ["lbm_capi.hip", "k_standard.hip", "k_reference.hip",
                "k_common.hip", "lbm_batch.hip", "k_batch.hip",
-               "lbm_serve.hip", "k_voxel.hip", "lbm_snapshot.hip"]
+               "lbm_serve.hip", "k_voxel.hip", "lbm_snapshot.hip",
+               "lbm_exports.hip"]
 HOST_SOURCES = ["Constants.cpp", "Geometry.cpp", "Utils.cpp",
                 "FlowCriterion.cpp", "Model.cpp", "Lattice.cpp"]
 CXX = os.environ.get("CXX", "g++")
@@ -142,6 +143,46 @@ def build_cli() -> str:
     return out
 
 
+SAN_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer",
+             "-fsanitize=address,undefined", "-fno-sanitize-recover=all"]
+
+
+def build_sanitized() -> list[str]:
+    """The host facade (host/*.cpp) and the C++ suites of tests/cpp built by
+    g++ under AddressSanitizer and UndefinedBehaviorSanitizer, every report
+    fatal: lib_san/liblbm.so and bin_san/<suite>.  They link the ordinary
+    liblbm_hip.so (the HIP layer is not instrumented; the suites the CPU
+    tests run make no GPU call).  tests/run_sanitized.sh runs the CPU tests
+    on them."""
+    libdir = os.path.join(PKG, "lib_san")
+    bindir = os.path.join(PKG, "bin_san")
+    objdir = os.path.join(libdir, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    os.makedirs(bindir, exist_ok=True)
+    flags = [f for f in CXX_FLAGS if f != "-O2"] + SAN_FLAGS
+    objs = [os.path.join(objdir, os.path.splitext(s)[0] + ".o")
+            for s in HOST_SOURCES]
+    with cf.ThreadPoolExecutor(len(HOST_SOURCES)) as ex:
+        list(ex.map(lambda so: _run([CXX, *flags, "-c", os.path.join(HOST, so[0]),
+                                     "-o", so[1]]),
+                    zip(HOST_SOURCES, objs)))
+    lib = os.path.join(libdir, "liblbm.so")
+    _run([CXX, "-shared", "-fopenmp", *SAN_FLAGS, *objs, f"-L{LIBDIR}",
+          "-llbm_hip", f"-Wl,-rpath,{LIBDIR}", "-o", lib + ".tmp"])
+    os.replace(lib + ".tmp", lib)
+
+    def one(name: str) -> str:
+        out = os.path.join(bindir, name)
+        _run([CXX, *flags, f"-I{TESTS_CPP}", os.path.join(TESTS_CPP, name + ".cpp"),
+              f"-L{libdir}", "-llbm", f"-L{LIBDIR}", "-llbm_hip",
+              f"-Wl,-rpath,{libdir}:{LIBDIR}", "-o", out + ".tmp"])
+        os.replace(out + ".tmp", out)
+        return out
+
+    with cf.ThreadPoolExecutor(len(CPP_SUITES)) as ex:
+        return [lib] + list(ex.map(one, CPP_SUITES))
+
+
 def build_all() -> list[str]:
     if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
@@ -152,5 +193,5 @@ def build_all() -> list[str]:
 
 
 if __name__ == "__main__":
-    for p in build_all():
+    for p in (build_sanitized() if "--sanitize" in sys.argv[1:] else build_all()):
         print(p)

@@ -23,6 +23,10 @@ D2Q9, D3Q19, D3Q27 = 9, 19, 27
 STORE_F32, STORE_F64, STORE_F16, STORE_BF16 = 0, 1, 2, 3
 MODE_STANDARD, MODE_REFERENCE = 0, 1
 ENGINE_BATCHED, ENGINE_PER_LATTICE = 0, 1
+ERR_INVALID, ERR_BUDGET, ERR_UNSTABLE = 1, 6, 8
+# serving request status
+SERVE_CONVERGED, SERVE_STEP_LIMIT, SERVE_REJECTED, SERVE_DIVERGED, \
+    SERVE_UNSTABLE = 0, 1, 2, 3, 4
 FLOW_NONE, FLOW_POINT, FLOW_PLANE, FLOW_BOX, FLOW_MASK = 0, 1, 2, 3, 4
 FLAG_INTERNAL, FLAG_REFLECTED = 1, 2
 MASK_SOLID = 1 << 31
@@ -99,6 +103,7 @@ class ServeParams(C.Structure):
         ("engine", C.c_int32),
         ("progress_s", C.c_double),
         ("max_resident_nodes", C.c_int64),
+        ("max_cell_re", C.c_double),
     ]
 
 
@@ -112,6 +117,9 @@ class ServeStats(C.Structure):
         ("completed", C.c_int32), ("rejected", C.c_int32),
         ("max_resident", C.c_int32),
         ("peak_bytes", C.c_int64),
+        ("unstable", C.c_int32),
+        ("diverged", C.c_int32),
+        ("population_bytes", C.c_double),
     ]
 
 
@@ -165,6 +173,8 @@ _SIGNATURES = {
     "lbm_batch_create": (C.c_int, [C.c_int, i64, P, C.POINTER(P)]),
     "lbm_batch_destroy": (C.c_int, [P]),
     "lbm_batch_estimate_bytes": (i64, [C.POINTER(Config), C.c_int]),
+    "lbm_cell_reynolds": (C.c_double, [C.POINTER(Config)]),
+    "lbm_batch_set_stability_limit": (C.c_int, [P, C.c_double]),
     "lbm_batch_add": (C.c_int, [P, C.POINTER(Config), P, C.c_int,
                                 C.POINTER(i32)]),
     "lbm_batch_remove": (C.c_int, [P, i32]),
@@ -546,6 +556,13 @@ class Batch:
         check(self.lib.lbm_batch_set_launch(self.h, block, grid),
               "lbm_batch_set_launch")
 
+    def set_stability_limit(self, max_cell_re: float):
+        """add() then raises LbmError (code ERR_UNSTABLE) for a lattice
+        whose cell Reynolds number exceeds max_cell_re (0: no limit)."""
+        check(self.lib.lbm_batch_set_stability_limit(self.h,
+                                                      float(max_cell_re)),
+              "lbm_batch_set_stability_limit")
+
     def launch(self):
         b, g = C.c_int(0), C.c_int(0)
         check(self.lib.lbm_batch_get_launch(self.h, C.byref(b), C.byref(g)),
@@ -583,6 +600,17 @@ class Batch:
         return {s: float(out[s]) for s in self.members}
 
 
+def cell_reynolds(cfg: Config) -> float:
+    """BGK cell Reynolds number U / nu of a standard-mode configuration
+    (lbm_cell_reynolds: U is the moving wall's speed or the Poiseuille
+    centre-line speed its body force drives across the narrowest walled
+    extent)."""
+    v = load().lbm_cell_reynolds(C.byref(cfg))
+    if v < 0:
+        check(int(-v), "lbm_cell_reynolds")
+    return float(v)
+
+
 def step_each(contexts, n: int) -> None:
     """n steps of every context, one single-lattice launch each per step."""
     arr = (CTX * len(contexts))(*[c.h for c in contexts])
@@ -591,12 +619,14 @@ def step_each(contexts, n: int) -> None:
 
 def serve(requests, storage=STORE_F16, budget_bytes=200 * 10 ** 9,
           chunk=100, max_resident=0, autotune=True, keep_results=False,
-          engine=ENGINE_BATCHED, progress_s=0.0, max_resident_nodes=0):
+          engine=ENGINE_BATCHED, progress_s=0.0, max_resident_nodes=0,
+          max_cell_re=0.0):
     """Runs lbm_serve_run on a list of dicts with keys cfg (Config), solid
     (array or None), arrival_s, max_steps, tol.  engine: ENGINE_BATCHED
     (persistent kernel, hipGraph chunks) or ENGINE_PER_LATTICE (one plain
-    launch per lattice per step).  Returns (stats dict, list of per-request
-    result dicts)."""
+    launch per lattice per step).  max_cell_re > 0 rejects requests above
+    that cell Reynolds number (status SERVE_UNSTABLE).  Returns (stats
+    dict, list of per-request result dicts)."""
     n = len(requests)
     arr = (Request * n)()
     keep = []
@@ -623,6 +653,7 @@ def serve(requests, storage=STORE_F16, budget_bytes=200 * 10 ** 9,
     p.engine = engine
     p.progress_s = progress_s
     p.max_resident_nodes = max_resident_nodes
+    p.max_cell_re = max_cell_re
     st = ServeStats()
     check(load().lbm_serve_run(C.byref(p), arr, n, C.byref(st)),
           "lbm_serve_run")

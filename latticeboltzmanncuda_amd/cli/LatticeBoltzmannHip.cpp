@@ -13,7 +13,11 @@
 // --quiet prints only the final force; --time adds one JSON line with the
 // main loop's wall time (the loop is synchronised before and after) and
 // the time of insertModel + preProcessModel (voxelisation and reflection
-// tables, on the GPU).
+// tables, on the GPU).  --read-records FILE replaces the starting
+// populations and flags with a record file (Lattice::readRecords) after
+// the model is preprocessed; --write-records FILE writes the final state
+// as one (Lattice::writeRecords: the 144-byte LatticePoint records of
+// retrieve_data after a 20-byte header).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -31,15 +35,20 @@ int main(int argc, char* argv[]) {
   if (argc < 12) {
     std::fprintf(stderr,
                  "usage: %s model.obj tx ty tz bx by bz density tau spacing steps "
-                 "[--staged] [--quiet] [--time]\n",
+                 "[--staged] [--quiet] [--time] [--read-records FILE] "
+                 "[--write-records FILE]\n",
                  argc > 0 ? argv[0] : "LatticeBoltzmannHip");
     return 1;
   }
   bool staged = false, quiet = false, timed = false;
+  const char* read_records = nullptr;
+  const char* write_records = nullptr;
   for (int a = 12; a < argc; ++a) {
     if (!std::strcmp(argv[a], "--staged")) staged = true;
     else if (!std::strcmp(argv[a], "--quiet")) quiet = true;
     else if (!std::strcmp(argv[a], "--time")) timed = true;
+    else if (!std::strcmp(argv[a], "--read-records") && a + 1 < argc) read_records = argv[++a];
+    else if (!std::strcmp(argv[a], "--write-records") && a + 1 < argc) write_records = argv[++a];
     else {
       std::fprintf(stderr, "unknown option %s\n", argv[a]);
       return 1;
@@ -68,6 +77,7 @@ int main(int argc, char* argv[]) {
     lattice.insertModel(argv[1]);
     lattice.preProcessModel();
     const double setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - s0).count();
+    if (read_records) lattice.readRecords(read_records);
 
     FlowData inducedFlow(&FlowCriterion::ptestPlaneCriteria, flowRef);
     lattice.setFlowData(&inducedFlow);
@@ -97,6 +107,7 @@ int main(int argc, char* argv[]) {
     lbm::check(lbm_synchronize(lattice.getDeviceHandle()), "lbm_synchronize");
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (quiet) print_force();
+    if (write_records) lattice.writeRecords(write_records);
     if (timed)
       std::printf("{\"nodes\": %lld, \"steps\": %d, \"seconds\": %.6f, \"mlups\": %.1f, "
                   "\"reflection_records\": %zu, \"setup_seconds\": %.4f, \"staged\": %s, "

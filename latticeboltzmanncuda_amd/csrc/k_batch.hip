@@ -62,9 +62,24 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return s;
 }
 
+// Maximum of non-negative values over the workgroup (thread 0's result).
+__device__ __forceinline__ double block_max(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s = fmax(s, red[w]);
+  __syncthreads();
+  return s;
+}
+
 // Velocity of the current state (u = (sum f e - F/2)/rho, fluid nodes),
-// compared with the snapshot, which it then replaces.  Writes per-chunk
-// sums of |u - u_prev|^2 and |u|^2.
+// compared with the snapshot, which it then replaces.  Gives the node's
+// |u - u_prev|^2 and |u|^2 for the per-chunk sums; a non-finite velocity
+// makes both non-finite, so the sums carry it.
 template <int Q, int S>
 __device__ __forceinline__ void residual_node(const BatchLattice& L, uint32_t n,
                                               double& d2, double& u2) {
@@ -122,11 +137,16 @@ __global__ void __launch_bounds__(kResidualChunk)
         break;
     }
   }
+  // largest |u|^2 of the chunk, +inf for a non-finite velocity (fmax
+  // would drop a NaN)
+  const double um = u2 <= 1e300 ? u2 : (double)INFINITY;
   const double sd = block_sum(d2, red);
   const double su = block_sum(u2, red);
+  const double sm = block_max(um, red);
   if (threadIdx.x == 0) {
-    partials[2 * c] = sd;
-    partials[2 * c + 1] = su;
+    partials[kResidualPartials * c] = sd;
+    partials[kResidualPartials * c + 1] = su;
+    partials[kResidualPartials * c + 2] = sm;
   }
 }
 

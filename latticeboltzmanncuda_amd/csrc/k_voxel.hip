@@ -290,12 +290,14 @@ __global__ void __launch_bounds__(kBlock)
 }  // namespace
 }  // namespace lbm
 
-using namespace lbm;
+// Entry points of include/lbm.h (exported by lbm_exports.hip).
+namespace lbm {
+namespace api {
 
-extern "C" int lbm_build_reflections(lbm_ctx* c, const float* vertices, int32_t nv,
-                                     const int32_t* faces, int32_t nf, float spacing,
-                                     int32_t max_records, int32_t* node, uint32_t* dirs,
-                                     float* weights, int32_t* count) {
+int lbm_build_reflections(lbm_ctx* c, const float* vertices, int32_t nv,
+                          const int32_t* faces, int32_t nf, float spacing,
+                          int32_t max_records, int32_t* node, uint32_t* dirs,
+                          float* weights, int32_t* count) {
   if (!c) return fail(LBM_ERR_INVALID, "null context");
   if (c->cfg.mode != LBM_MODE_REFERENCE)
     return fail(LBM_ERR_STATE, "lbm_build_reflections needs a reference-mode lattice");
@@ -376,9 +378,9 @@ extern "C" int lbm_build_reflections(lbm_ctx* c, const float* vertices, int32_t 
   return LBM_OK;
 }
 
-extern "C" int lbm_voxelize_mesh(lbm_ctx* c, const float* vertices, int32_t nv,
-                                 const int32_t* faces, int32_t nf, float spacing,
-                                 uint8_t* inside, int64_t* count) {
+int lbm_voxelize_mesh(lbm_ctx* c, const float* vertices, int32_t nv,
+                      const int32_t* faces, int32_t nf, float spacing,
+                      uint8_t* inside, int64_t* count) {
   if (!c) return fail(LBM_ERR_INVALID, "null context");
   if (!inside || nv < 0 || nf < 0 || (nf > 0 && (!vertices || !faces)))
     return fail(LBM_ERR_INVALID, "bad mesh arguments");
@@ -445,3 +447,6 @@ extern "C" int lbm_voxelize_mesh(lbm_ctx* c, const float* vertices, int32_t nv,
   }
   return LBM_OK;
 }
+
+}  // namespace api
+}  // namespace lbm

@@ -42,6 +42,8 @@ struct BatchTile {
 // Residuals are summed over fixed 256-node chunks, so a lattice's residual
 // does not depend on the launch shape or on the other members.
 constexpr uint32_t kResidualChunk = 256;
+// Per residual chunk: sum |u - u_prev|^2, sum |u|^2, max |u|^2.
+constexpr uint32_t kResidualPartials = 3;
 
 // Tiles are kept in one table per kernel class (lattice type), and each
 // step launches one persistent kernel per non-empty class: each kernel
@@ -70,6 +72,7 @@ struct lbm_batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int device_cus = 0;
 
+  double max_cell_re = 0.0;        // admission stability limit, 0 = none
   std::vector<lbm_ctx*> slots;     // nullptr = free slot
   std::vector<float*> snapshots;   // per slot, 3 x N floats (or nullptr)
   std::vector<int64_t> slot_bytes;
@@ -105,6 +108,8 @@ struct lbm_batch {
   hipGraphExec_t graph_exec = nullptr;
   int graph_chunk = 0, graph_block = 0;
   bool graph_nt = false;
+  const void* graph_lats = nullptr;    // descriptor table the graph reads
+  const void* graph_ntiles = nullptr;  // tile counts the graph reads
   // per class: tile table and grid the captured launches use (null / 0 for
   // a class that was empty)
   void* graph_tables[lbm::kBatchClasses] = {};

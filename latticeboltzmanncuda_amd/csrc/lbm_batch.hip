@@ -44,20 +44,24 @@ int grow(lbm_batch* b, T** p, uint32_t* cap, uint32_t need, const char* what) {
   if (need <= *cap && *p) return LBM_OK;
   uint32_t c = std::max<uint32_t>(need, 64);
   c = c + c / 2;
+  // captured launches point at the old table: the graph goes first (after
+  // its replays finish), so no instantiated graph ever references freed
+  // memory, whether or not the new allocation succeeds
+  drop_graph(b);
   if (*p) {
     (void)hipStreamSynchronize(b->stream);
     (void)hipFree(*p);
     b->bytes -= (int64_t)(*cap) * (int64_t)sizeof(T);
   }
   *p = nullptr;
+  *cap = 0;
   hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)c);
   if (e != hipSuccess) {
-    *cap = 0;
+    *p = nullptr;
     return hip_fail(e, what);
   }
   b->bytes += (int64_t)c * (int64_t)sizeof(T);
   *cap = c;
-  drop_graph(b);  // captured launches point at the old tables
   return LBM_OK;
 }
 
@@ -119,16 +123,17 @@ int sync_tables(lbm_batch* b) {
     if (b->d_partials) {
       (void)hipStreamSynchronize(b->stream);
       (void)hipFree(b->d_partials);
-      b->bytes -= (int64_t)sizeof(double) * 2 * b->partials_cap;
+      b->bytes -= (int64_t)sizeof(double) * kResidualPartials * b->partials_cap;
     }
     b->partials_cap = 0;
-    hipError_t e = hipMalloc((void**)&b->d_partials, sizeof(double) * 2 * b->chunks_cap);
+    hipError_t e =
+        hipMalloc((void**)&b->d_partials, sizeof(double) * kResidualPartials * b->chunks_cap);
     if (e != hipSuccess) {
       b->d_partials = nullptr;
       return hip_fail(e, "hipMalloc(batch partials)");
     }
     b->partials_cap = b->chunks_cap;
-    b->bytes += (int64_t)sizeof(double) * 2 * b->partials_cap;
+    b->bytes += (int64_t)sizeof(double) * kResidualPartials * b->partials_cap;
   }
   if (!b->d_ntiles) {
     hipError_t e = hipMalloc((void**)&b->d_ntiles, sizeof(uint32_t) * kBatchClasses);
@@ -213,6 +218,8 @@ int capture_chunk(lbm_batch* b, int chunk) {
   b->graph_chunk = chunk;
   b->graph_block = b->block;
   b->graph_nt = b->nt;
+  b->graph_lats = b->d_lats;
+  b->graph_ntiles = b->d_ntiles;
   for (int k = 0; k < kBatchClasses; ++k) {
     b->graph_grids[k] = class_grid(b, k);
     b->graph_tables[k] = b->graph_grids[k] ? (void*)b->d_tiles[k] : nullptr;
@@ -221,10 +228,13 @@ int capture_chunk(lbm_batch* b, int chunk) {
 }
 
 // The captured launches stay valid while every class launches the same
-// grid from the same tile table (the tables' contents may change).
+// grid from the same tables (the tables' contents may change).  A table
+// is only ever freed after the graph is dropped (grow), so an address
+// that matches here is the live table the graph was captured with, never
+// a freed one that happens to be reused.
 bool graph_current(const lbm_batch* b, int chunk) {
   if (!b->graph_exec || b->graph_chunk != chunk || b->graph_block != b->block ||
-      b->graph_nt != b->nt)
+      b->graph_nt != b->nt || b->graph_lats != b->d_lats || b->graph_ntiles != b->d_ntiles)
     return false;
   for (int k = 0; k < kBatchClasses; ++k) {
     const int grid = class_grid(b, k);
@@ -259,7 +269,9 @@ std::tuple<int, int, int> tune_key(const lbm_batch* b) {
 
 }  // namespace
 
-extern "C" {
+// Entry points of include/lbm.h (exported by lbm_exports.hip).
+namespace lbm {
+namespace api {
 
 int lbm_batch_create(int storage, int64_t budget_bytes, void* stream,
                      lbm_batch** out) {
@@ -330,6 +342,14 @@ int lbm_batch_add(lbm_batch* b, const lbm_config* cfg, const uint8_t* solid,
     return fail(LBM_ERR_UNSUPPORTED, "batches hold standard-mode lattices");
   if (cfg->storage != b->storage)
     return fail(LBM_ERR_INVALID, "lattice storage differs from the batch storage");
+  if (int rc = check_config(cfg)) return rc;
+  if (b->max_cell_re > 0.0) {
+    const double re = cell_reynolds(*cfg);
+    if (!(re <= b->max_cell_re))
+      return fail(LBM_ERR_UNSTABLE, "cell Reynolds number " + std::to_string(re) +
+                                        " above the batch's stability limit " +
+                                        std::to_string(b->max_cell_re));
+  }
   const int64_t need = member_bytes(*cfg, track_residual);
   if (b->bytes + need > b->budget)
     return fail(LBM_ERR_BUDGET, "HBM budget of the batch would be exceeded");
@@ -369,6 +389,13 @@ int lbm_batch_add(lbm_batch* b, const lbm_config* cfg, const uint8_t* solid,
   b->bytes += need;
   b->dirty = true;
   *slot = s;
+  return LBM_OK;
+}
+
+int lbm_batch_set_stability_limit(lbm_batch* b, double max_cell_re) {
+  if (int rc = check_batch(b)) return rc;
+  if (!(max_cell_re >= 0.0)) return fail(LBM_ERR_INVALID, "stability limit must be >= 0");
+  b->max_cell_re = max_cell_re;
   return LBM_OK;
 }
 
@@ -471,7 +498,7 @@ int lbm_batch_time_steps(lbm_batch* b, int n, int chunk, float* ms) {
   if (rc) return rc;
   if (chunk > 0 && b->ntiles > 0 && (rc = ensure_graph(b, chunk))) return rc;
   LBM_HIP(hipEventRecord(b->ev0, b->stream));
-  rc = chunk > 0 ? lbm_batch_step_graph(b, n, chunk) : lbm_batch_step(b, n);
+  rc = chunk > 0 ? api::lbm_batch_step_graph(b, n, chunk) : api::lbm_batch_step(b, n);
   if (rc) return rc;
   LBM_HIP(hipEventRecord(b->ev1, b->stream));
   LBM_HIP(hipEventSynchronize(b->ev1));
@@ -548,22 +575,30 @@ int lbm_batch_residuals(lbm_batch* b, double* out, int32_t nslots) {
                                   b->d_chunks, b->nchunks,
                                   b->d_partials, b->stream)))
     return rc;
-  std::vector<double> part(2 * (size_t)b->nchunks);
+  std::vector<double> part(kResidualPartials * (size_t)b->nchunks);
   LBM_HIP(hipMemcpyAsync(part.data(), b->d_partials, sizeof(double) * part.size(),
                          hipMemcpyDeviceToHost, b->stream));
   LBM_HIP(hipStreamSynchronize(b->stream));
+  const double limit2 = LBM_DIVERGED_SPEED * LBM_DIVERGED_SPEED;
   for (size_t k = 0; k < b->h_lats.size(); ++k) {
     const int32_t s = b->h_lat_slot[k];
     if (!b->snapshots[s]) continue;
-    double d2 = 0.0, u2 = 0.0;
+    double d2 = 0.0, u2 = 0.0, m2 = 0.0;
     const uint32_t c0 = b->h_lats[k].res_first;
     for (uint32_t c = c0; c < c0 + b->h_res_count[k]; ++c) {
-      d2 += part[2 * c];
-      u2 += part[2 * c + 1];
+      d2 += part[kResidualPartials * c];
+      u2 += part[kResidualPartials * c + 1];
+      m2 = std::max(m2, part[kResidualPartials * c + 2]);
     }
-    out[s] = u2 > 0.0 ? std::sqrt(d2 / u2) : (d2 > 0.0 ? INFINITY : 0.0);
+    // a non-finite velocity anywhere, or a speed beyond any stable BGK
+    // flow, marks the lattice as diverged
+    if (!std::isfinite(d2) || !std::isfinite(u2) || !(m2 <= limit2))
+      out[s] = NAN;
+    else
+      out[s] = u2 > 0.0 ? std::sqrt(d2 / u2) : (d2 > 0.0 ? INFINITY : 0.0);
   }
   return LBM_OK;
 }
 
-}  // extern "C"
+}  // namespace api
+}  // namespace lbm

@@ -8,8 +8,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <random>
 #include <string>
 #include <vector>
@@ -33,6 +36,26 @@ int fail(int code, const std::string& msg) {
 int hip_fail(hipError_t e, const char* what) {
   g_last_error = std::string(what) + ": " + hipGetErrorString(e);
   return e == hipErrorOutOfMemory ? LBM_ERR_NOMEM : LBM_ERR_HIP;
+}
+
+int api_exception() noexcept {
+  int code = LBM_ERR_INVALID;
+  const char* msg = "internal error: unknown exception";
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    code = LBM_ERR_NOMEM;
+    msg = "host allocation failed";
+  } catch (const std::exception& e) {
+    msg = e.what();
+  } catch (...) {
+  }
+  try {
+    g_last_error = msg;
+  } catch (...) {
+    // no memory for the message either: keep the previous one
+  }
+  return code;
 }
 
 namespace {
@@ -96,6 +119,27 @@ StdParams make_std_params(const lbm_config& cfg) {
       break;
   }
   return p;
+}
+
+double cell_reynolds(const lbm_config& cfg) {
+  const double nu = ((double)cfg.tau - 0.5) / 3.0;
+  const double uw = std::sqrt((double)cfg.lid_u[0] * cfg.lid_u[0] +
+                              (double)cfg.lid_u[1] * cfg.lid_u[1] +
+                              (double)cfg.lid_u[2] * cfg.lid_u[2]);
+  const double F = std::sqrt((double)cfg.force[0] * cfg.force[0] +
+                             (double)cfg.force[1] * cfg.force[1] +
+                             (double)cfg.force[2] * cfg.force[2]);
+  // narrowest walled extent (axes of one node carry no flow)
+  const int32_t ext[3] = {cfg.nx, cfg.ny, cfg.nz};
+  int32_t walled = 0, widest = 1;
+  for (int a = 0; a < 3; ++a) {
+    if (ext[a] < 2) continue;
+    widest = std::max(widest, ext[a]);
+    if (!cfg.periodic[a]) walled = walled ? std::min(walled, ext[a]) : ext[a];
+  }
+  const double W = walled ? walled : widest;
+  const double U = std::max(cfg.lid_face >= 0 ? uw : 0.0, F * W * W / (8.0 * nu));
+  return U / nu;
 }
 
 int64_t estimate_ctx_bytes(const lbm_config& cfg) {
@@ -209,6 +253,10 @@ int validate_config(const lbm_config* cfg) {
     return fail(LBM_ERR_INVALID, "tau must be positive");
   if (cfg->mode == LBM_MODE_STANDARD && !(cfg->tau > 0.5f))
     return fail(LBM_ERR_INVALID, "standard BGK needs tau > 1/2");
+  for (int a = 0; a < 3; ++a)
+    if (!std::isfinite(cfg->force[a]) || !std::isfinite(cfg->lid_u[a]))
+      return fail(LBM_ERR_INVALID, "body force and wall velocity must be finite");
+  if (!std::isfinite(cfg->wall_rho)) return fail(LBM_ERR_INVALID, "wall density must be finite");
   if (cfg->lid_face < -1 || cfg->lid_face > 5)
     return fail(LBM_ERR_INVALID, "lid_face must be -1..5");
   if (cfg->lid_face >= 0) {
@@ -462,15 +510,34 @@ int step_reference_staged(lbm_ctx* c, int n, int with_force) {
 
 int check_config(const lbm_config* cfg) { return validate_config(cfg); }
 
+int read_moments(lbm_ctx* c, void* stage, float* rho, float* u) {
+  const size_t N = c->grid.n;
+  float* dr = rho ? (float*)stage : nullptr;
+  float* du = u ? (float*)stage + (rho ? N : 0) : nullptr;
+  int rc;
+  if (c->cfg.mode == LBM_MODE_REFERENCE)
+    rc = launch_ref_moments(c->grid, (const float*)c->f[c->cur], dr, du, c->stream);
+  else
+    rc = launch_std_moments(c->q, device_storage(c->cfg.mode, c->cfg.storage), c->grid,
+                            c->f[c->cur], c->mask, c->cfg.force, dr, du, c->stream);
+  if (rc) return rc;
+  if (rho && (rc = d2h(c, rho, dr, sizeof(float) * N))) return rc;
+  if (u && (rc = d2h(c, u, du, sizeof(float) * 3 * N))) return rc;
+  return sync(c);
+}
+
 }  // namespace lbm
 
 using namespace lbm;
 
-extern "C" {
+extern "C" const char* lbm_last_error(void) { return g_last_error.c_str(); }
 
-const char* lbm_last_error(void) { return g_last_error.c_str(); }
+extern "C" int lbm_abi_version(void) { return LBM_ABI_VERSION; }
 
-int lbm_abi_version(void) { return LBM_ABI_VERSION; }
+// The entry points of include/lbm.h; lbm_exports.hip exports each one
+// with C linkage behind the exception boundary.
+namespace lbm {
+namespace api {
 
 int lbm_device_info(int* device_count, char* name, size_t name_len) {
   int n = 0;
@@ -570,7 +637,7 @@ int lbm_destroy(lbm_ctx* c) {
                   (void*)c->ref_cls, (void*)c->ref_inmask, (void*)c->ref_rxbase,
                   (void*)c->ref_gsrc, (void*)c->ref_gbeg, (void*)c->ref_gsum,
                   (void*)c->ref_eidx, (void*)c->ref_ew, (void*)c->ref_es,
-                  (void*)c->ref_partials})
+                  (void*)c->ref_partials, c->stage})
     if (p) (void)hipFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -590,6 +657,13 @@ int64_t lbm_node_count(const lbm_ctx* c) { return c ? (int64_t)c->grid.n : -1; }
 
 int64_t lbm_device_bytes(const lbm_ctx* c) { return c ? c->device_bytes : -1; }
 
+double lbm_cell_reynolds(const lbm_config* cfg) {
+  if (int rc = validate_config(cfg)) return -(double)rc;
+  if (cfg->mode != LBM_MODE_STANDARD)
+    return -(double)fail(LBM_ERR_STATE, "lbm_cell_reynolds needs a standard-mode configuration");
+  return cell_reynolds(*cfg);
+}
+
 int lbm_set_tau(lbm_ctx* c, float tau) {
   if (int rc = check_ctx(c)) return rc;
   if (!(tau > 0.0f) || (c->cfg.mode == LBM_MODE_STANDARD && !(tau > 0.5f)))
@@ -602,6 +676,8 @@ int lbm_set_tau(lbm_ctx* c, float tau) {
 int lbm_set_body_force(lbm_ctx* c, const float force[3]) {
   if (int rc = check_ctx(c)) return rc;
   if (!force) return fail(LBM_ERR_INVALID, "null force");
+  for (int a = 0; a < 3; ++a)
+    if (!std::isfinite(force[a])) return fail(LBM_ERR_INVALID, "body force must be finite");
   for (int a = 0; a < 3; ++a) c->cfg.force[a] = force[a];
   ++c->params_version;
   return LBM_OK;
@@ -893,7 +969,7 @@ int lbm_time_steps(lbm_ctx* c, int n, float* ms) {
   if (int rc = check_ctx(c)) return rc;
   if (!ms) return fail(LBM_ERR_INVALID, "null output");
   LBM_HIP(hipEventRecord(c->ev0, c->stream));
-  int rc = lbm_step(c, n);
+  int rc = api::lbm_step(c, n);
   if (rc) return rc;
   LBM_HIP(hipEventRecord(c->ev1, c->stream));
   LBM_HIP(hipEventSynchronize(c->ev1));
@@ -907,7 +983,7 @@ int lbm_time_steps_reference(lbm_ctx* c, int n, int with_force, float* ms) {
   int rc = rebuild_special(c);
   if (rc) return rc;
   LBM_HIP(hipEventRecord(c->ev0, c->stream));
-  if ((rc = lbm_step_reference(c, n, with_force))) return rc;
+  if ((rc = api::lbm_step_reference(c, n, with_force))) return rc;
   LBM_HIP(hipEventRecord(c->ev1, c->stream));
   LBM_HIP(hipEventSynchronize(c->ev1));
   LBM_HIP(hipEventElapsedTime(ms, c->ev0, c->ev1));
@@ -917,21 +993,16 @@ int lbm_time_steps_reference(lbm_ctx* c, int n, int with_force, float* ms) {
 int lbm_moments(lbm_ctx* c, float* rho, float* u) {
   if (int rc = check_ctx(c)) return rc;
   const size_t N = c->grid.n;
-  DevBuf dr, du;
   int rc;
-  if (rho && (rc = dr.alloc(sizeof(float) * N))) return rc;
-  if (u && (rc = du.alloc(sizeof(float) * 3 * N))) return rc;
-  if (c->cfg.mode == LBM_MODE_REFERENCE)
-    rc = launch_ref_moments(c->grid, (const float*)c->f[c->cur],
-                            (float*)dr.p, (float*)du.p, c->stream);
-  else
-    rc = launch_std_moments(c->q, device_storage(c->cfg.mode, c->cfg.storage), c->grid, c->f[c->cur],
-                            c->mask, c->cfg.force, (float*)dr.p, (float*)du.p,
-                            c->stream);
-  if (rc) return rc;
-  if (rho && (rc = d2h(c, rho, dr.p, sizeof(float) * N))) return rc;
-  if (u && (rc = d2h(c, u, du.p, sizeof(float) * 3 * N))) return rc;
-  return sync(c);
+  // rho and u staged in the context's buffer (4 N + 12 N bytes), kept
+  // from call to call
+  if (c->stage_bytes < 16 * N) {
+    dev_free(c, &c->stage, c->stage_bytes);
+    c->stage_bytes = 0;
+    if ((rc = dev_alloc(c, &c->stage, 16 * N, "hipMalloc(moments staging)"))) return rc;
+    c->stage_bytes = 16 * N;
+  }
+  return read_moments(c, c->stage, rho, u);
 }
 
 int lbm_conservation(lbm_ctx* c, double out[4]) {
@@ -1055,4 +1126,5 @@ int64_t lbm_random_spheres(int32_t nx, int32_t ny, int32_t nz, uint64_t seed,
   return ns;
 }
 
-}  // extern "C"
+}  // namespace api
+}  // namespace lbm

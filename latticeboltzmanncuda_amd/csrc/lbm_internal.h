@@ -83,11 +83,23 @@ StdParams make_std_params(const lbm_config& cfg);
 int check_config(const lbm_config* cfg);
 // device bytes lbm_create allocates for cfg (populations, geometry, scratch)
 int64_t estimate_ctx_bytes(const lbm_config& cfg);
+// BGK cell Reynolds number U / nu of a standard-mode configuration (the
+// rule of lbm_cell_reynolds); cfg must pass check_config.
+double cell_reynolds(const lbm_config& cfg);
+// The moments of lbm_moments (rho and / or u, either may be null) staged
+// through the device buffer `stage` -- 4 N bytes for rho, then 12 N for u
+// (12 N from the start when rho is null) -- and copied to the host.
+int read_moments(lbm_ctx* c, void* stage, float* rho, float* u);
 
 // error helpers --------------------------------------------------------
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
+// Status of an exception caught at the C-ABI boundary, from inside the
+// catch block (std::bad_alloc -> LBM_ERR_NOMEM, anything else ->
+// LBM_ERR_INVALID), with the message set when the message itself can be
+// stored.
+int api_exception() noexcept;
 
 #define LBM_HIP(call)                                     \
   do {                                                    \
@@ -217,6 +229,10 @@ struct lbm_ctx {
   double* red = nullptr;
   int red_cap = 0;
   double* red_out = nullptr;
+  // device staging of lbm_moments (rho and u), allocated by its first call
+  // and kept, so later calls allocate nothing
+  void* stage = nullptr;
+  size_t stage_bytes = 0;
 
   hipStream_t stream = nullptr;
   bool own_stream = false;

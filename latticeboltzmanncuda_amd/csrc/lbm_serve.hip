@@ -32,6 +32,7 @@
 
 #include "lbm_batch.h"
 #include "lbm_internal.h"
+#include "lbm_store.h"
 
 using namespace lbm;
 
@@ -51,7 +52,30 @@ double percentile(std::vector<double> v, double p) {
   return v[lo] + (v[hi] - v[lo]) * (pos - (double)lo);
 }
 
+// Device staging of retirement readbacks, freed when the server returns
+// (each readback synchronises before the next one reuses it).
+struct StageBuffer {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~StageBuffer() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int grow(size_t n) {
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return hip_fail(e, "hipMalloc(serving readback staging)");
+    }
+    bytes = n;
+    return LBM_OK;
+  }
+};
+
 int validate_request(const lbm_request& r, int storage) {
+  if (int rc = check_config(&r.cfg)) return rc;
   if (r.cfg.mode != LBM_MODE_STANDARD)
     return fail(LBM_ERR_UNSUPPORTED, "requests must be standard-mode lattices");
   if (r.cfg.storage != storage)
@@ -82,14 +106,19 @@ int pretune(const lbm_serve_params& p, const lbm_request* reqs, int32_t n,
 
 }  // namespace
 
-extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
-                             int32_t nreq, lbm_serve_stats* stats) {
+// Entry point of include/lbm.h (exported by lbm_exports.hip).
+namespace lbm {
+namespace api {
+
+int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs, int32_t nreq,
+                  lbm_serve_stats* stats) {
   if (!p || (!reqs && nreq > 0) || nreq < 0 || !stats)
     return fail(LBM_ERR_INVALID, "null argument");
   if (p->chunk < 2 || (p->chunk & 1))
     return fail(LBM_ERR_INVALID, "chunk must be an even step count >= 2");
   if (p->engine != LBM_ENGINE_BATCHED && p->engine != LBM_ENGINE_PER_LATTICE)
     return fail(LBM_ERR_INVALID, "unknown serving engine");
+  if (!(p->max_cell_re >= 0.0)) return fail(LBM_ERR_INVALID, "max_cell_re must be >= 0");
   for (int32_t k = 0; k < nreq; ++k) {
     if (int rc = validate_request(reqs[k], p->storage)) return rc;
     reqs[k].t_admit_s = reqs[k].t_done_s = -1.0;
@@ -114,8 +143,16 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
   std::vector<int32_t> slot_req;  // slot -> request (-1 free)
   int32_t resident = 0, next = 0, finished = 0;
   int64_t resident_nodes = 0;
+  // algorithmic bytes one step of the resident lattices moves: 2 Q s per
+  // node (every population read once and written once)
+  double resident_step_bytes = 0.0, pop_bytes = 0.0;
+  const double elem = (double)storage_bytes(p->storage);
+  auto step_bytes = [&](const lbm_config& cfg) {
+    return (double)cfg.nx * cfg.ny * cfg.nz * 2.0 * cfg.lattice * elem;
+  };
   std::vector<double> residuals;
   std::vector<float> scratch;
+  StageBuffer stage;
   std::vector<lbm_ctx*> members;
   int64_t updates = 0;
   const auto t0 = Clock::now();
@@ -132,7 +169,14 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
       scratch.resize(n3);
       dst = scratch.data();
     }
-    if ((e = lbm_moments(c, nullptr, dst))) return e;
+    // the velocity is staged through one device buffer of the server,
+    // grown to the largest lattice retired so far (no allocation per
+    // request)
+    if (stage.bytes < sizeof(float) * n3) {
+      stage.release();
+      if ((e = stage.grow(sizeof(float) * n3))) return e;
+    }
+    if ((e = read_moments(c, stage.p, nullptr, dst))) return e;
     reqs[r].t_done_s = seconds_since(t0);
     reqs[r].residual = res;
     reqs[r].status = status;
@@ -141,6 +185,7 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
     slot_req[slot] = -1;
     --resident;
     resident_nodes -= (int64_t)reqs[r].cfg.nx * reqs[r].cfg.ny * reqs[r].cfg.nz;
+    resident_step_bytes -= step_bytes(reqs[r].cfg);
     ++finished;
     return LBM_OK;
   };
@@ -158,6 +203,15 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
     // admission, FIFO
     while (!fifo.empty()) {
       const int32_t r = fifo.front();
+      if (p->max_cell_re > 0.0 && !(cell_reynolds(reqs[r].cfg) <= p->max_cell_re)) {
+        // a BGK flow this far beyond the stability limit would diverge
+        reqs[r].status = 4;
+        reqs[r].t_done_s = seconds_since(t0);
+        fifo.pop_front();
+        ++finished;
+        ++stats->unstable;
+        continue;
+      }
       const int64_t need = lbm_batch_estimate_bytes(&reqs[r].cfg, 1);
       if (need > p->budget_bytes) {
         reqs[r].status = 2;
@@ -196,6 +250,7 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
       fifo.pop_front();
       ++resident;
       resident_nodes += nodes;
+      resident_step_bytes += step_bytes(reqs[r].cfg);
       stats->max_resident = std::max(stats->max_resident, resident);
       int64_t bytes = 0;
       lbm_batch_info(b, nullptr, nullptr, &bytes);
@@ -227,6 +282,7 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
       return rc;
     }
     updates += nodes * (int64_t)p->chunk;
+    pop_bytes += resident_step_bytes * p->chunk;
     residuals.assign(slot_req.size(), -1.0);
     if ((rc = lbm_batch_residuals(b, residuals.data(), (int32_t)residuals.size()))) {
       lbm_batch_destroy(b);
@@ -239,8 +295,11 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
       const double res = residuals[s];
       const bool converged = res >= 0.0 && res < reqs[r].tol;
       const bool limit = reqs[r].steps >= reqs[r].max_steps;
+      // a diverged lattice reads NaN (lbm_batch_residuals): retired now,
+      // the other lattices' arithmetic does not depend on it
       if (converged || limit || !std::isfinite(res)) {
         const int32_t status = converged ? 0 : (std::isfinite(res) ? 1 : 3);
+        if (status == 3) ++stats->diverged;
         if ((rc = retire(s, status, res))) {
           lbm_batch_destroy(b);
           return rc;
@@ -256,6 +315,7 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
   stats->completed = (int32_t)lat.size();
   stats->wall_s = t_first < 0.0 ? 0.0 : t_last - t_first;
   stats->lattice_updates = updates;
+  stats->population_bytes = pop_bytes;
   stats->mlups = stats->wall_s > 0.0 ? (double)updates / stats->wall_s / 1e6 : 0.0;
   stats->p50_latency_s = percentile(lat, 0.50);
   stats->p99_latency_s = percentile(lat, 0.99);
@@ -264,3 +324,6 @@ extern "C" int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs,
   stats->mean_latency_s = lat.empty() ? 0.0 : sum / (double)lat.size();
   return LBM_OK;
 }
+
+}  // namespace api
+}  // namespace lbm

@@ -82,9 +82,42 @@ int upload(lbm_ctx* c, void* dst, const void* src, size_t n) {
   return LBM_OK;
 }
 
+// Bytes of the file after the header, or -1 when its size is unknown.
+int64_t bytes_after_header(FILE* f) {
+  const long here = std::ftell(f);
+  if (here < 0 || std::fseek(f, 0, SEEK_END) != 0) return -1;
+  const long end = std::ftell(f);
+  if (end < 0 || std::fseek(f, here, SEEK_SET) != 0) return -1;
+  return (int64_t)end - (int64_t)here;
+}
+
+// The checks that lbm_upload_reflections, lbm_set_flow and the receiver
+// tables of the fused reference step apply to the same data, made before
+// the context changes: every record's node in range and recorded once, a
+// reflecting node with a record inside the obstacle, a known flow kind.
+int check_reference_sections(uint64_t N, const std::vector<uint8_t>& flags,
+                             const std::vector<int32_t>& rnode, const SnapHeader& h) {
+  std::vector<uint8_t> seen(N, 0);
+  for (const int32_t node : rnode) {
+    if (node < 0 || (uint64_t)node >= N)
+      return fail(LBM_ERR_INVALID, "snapshot reflection record node out of range");
+    if (seen[node]) return fail(LBM_ERR_INVALID, "snapshot holds two reflection records for a node");
+    seen[node] = 1;
+    if ((flags[node] & LBM_FLAG_REFLECTED) && !(flags[node] & LBM_FLAG_INTERNAL))
+      return fail(LBM_ERR_INVALID, "snapshot reflecting node is not internal");
+  }
+  if (h.flow_kind < LBM_FLOW_NONE || h.flow_kind > LBM_FLOW_MASK)
+    return fail(LBM_ERR_INVALID, "snapshot flow kind unknown");
+  if (h.flow_kind == LBM_FLOW_PLANE && (h.flow_params[0] < 0 || h.flow_params[0] > 2))
+    return fail(LBM_ERR_INVALID, "snapshot flow plane axis must be 0..2");
+  return LBM_OK;
+}
+
 }  // namespace
 
-extern "C" {
+// Entry points of include/lbm.h (exported by lbm_exports.hip).
+namespace lbm {
+namespace api {
 
 int lbm_snapshot_save(lbm_ctx* c, const char* path) {
   if (!c || !path) return fail(LBM_ERR_INVALID, "null argument");
@@ -165,11 +198,22 @@ int lbm_snapshot_load(lbm_ctx* c, const char* path) {
   if (int e = check_config(&s)) return e;
   const uint64_t N = c->grid.n;
   const bool standard = c->cfg.mode == LBM_MODE_STANDARD;
+  // a node holds at most one reflection record, so nrec <= N; standard
+  // lattices have none
   if (h.pop_bytes != population_elems(c->q, N) * storage_bytes(c->cfg.storage) ||
       h.mask_bytes != (standard ? 4 * N : 0) || h.cls_bytes != (standard ? 4 * ((N + 15) / 16) : 0) ||
-      h.flag_bytes != (standard ? 0 : N) || h.nrec < 0 ||
+      h.flag_bytes != (standard ? 0 : N) || h.nrec < 0 || (uint64_t)h.nrec > N ||
+      (standard && h.nrec != 0) ||
       h.flow_mask_bytes != ((!standard && h.flow_kind == LBM_FLOW_MASK) ? N : 0))
     return fail(LBM_ERR_INVALID, "snapshot sections do not match the context");
+  // the sections must fill the rest of the file exactly: checked before
+  // any buffer is sized from the header
+  const uint64_t rec_bytes = (uint64_t)h.nrec * (4 + 4 * 14 + 4 * 81);
+  const uint64_t body = h.pop_bytes + h.mask_bytes + h.cls_bytes + h.flag_bytes + rec_bytes +
+                        h.flow_mask_bytes;
+  const int64_t left = bytes_after_header(in.f);
+  if (left < 0 || (uint64_t)left != body)
+    return fail(LBM_ERR_INVALID, "snapshot file size does not match its header");
   std::vector<uint8_t> pops(h.pop_bytes), mask, cls, flags, fmask;
   std::vector<int32_t> rnode;
   std::vector<uint32_t> rdirs;
@@ -194,6 +238,7 @@ int lbm_snapshot_load(lbm_ctx* c, const char* path) {
       fmask.resize(N);
       if ((rc = read_all(in.f, fmask.data(), N))) return rc;
     }
+    if ((rc = check_reference_sections(N, flags, rnode, h))) return rc;
   }
   // everything read and checked: now change the context
   c->cfg.tau = s.tau;
@@ -225,4 +270,5 @@ int lbm_snapshot_load(lbm_ctx* c, const char* path) {
   return LBM_OK;
 }
 
-}  // extern "C"
+}  // namespace api
+}  // namespace lbm

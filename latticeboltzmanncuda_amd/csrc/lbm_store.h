@@ -2,7 +2,9 @@
 // shifted deviation f_i - w_i (the rest-state value w_i carries most of the
 // magnitude, so the 16-bit mantissa spends its bits on the deviation).  F16
 // holds the deviation times 2^12 (exact both ways), so deviations down to
-// 1.5e-8 are normal FP16 numbers; |f_i - w_i| up to 16 fits.
+// 1.5e-8 are normal FP16 numbers; |f_i - w_i| < 65504 / 2^12 ~ 15.99 fits
+// (a larger deviation is stored as +-Inf, which the residual and
+// conservation checks then see).
 //
 // Conversions to 16 bits round to nearest, ties to even -- or, through
 // store_sr, stochastically: a flow driven by a small body force changes its
@@ -148,9 +150,10 @@ struct Store<LBM_STORE_BF16> {
   __device__ static T rest(int, double) { return 0; }
   __device__ static C load(const T* p) { return cvt(gload(p)); }
   __device__ static T round(C v) {
-    uint32_t u = __float_as_uint(v);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+    const uint32_t u = __float_as_uint(v);
+    // saturating add, as in round_sr: a NaN pattern near 0x7FFFFFFF or
+    // 0xFFFFFFFF would otherwise wrap round to a signed zero
+    return (uint16_t)(__builtin_elementwise_add_sat(u, 0x7fffu + ((u >> 16) & 1u)) >> 16);
   }
   __device__ static T round_sr(C v, uint32_t key) {
     const uint32_t u = __float_as_uint(v);

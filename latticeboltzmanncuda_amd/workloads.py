@@ -189,6 +189,24 @@ def c4_specs(count: int = 512, seed: int = 4, smin: int = 32, smax: int = 512,
     return out, clamped
 
 
+# Admission bound of the batching workload on the BGK cell Reynolds number
+# U / nu (lbm_cell_reynolds).  Measured on the 512 lattices of seed 4 after
+# 1000 steps: every lattice whose FP64 run diverges (21 lid-driven
+# cavities) has U / nu >= 17.4, and FP16 storage departs from FP64 by
+# 1e-3 to 1.7e-1 on the near-unstable cavities at 14-19; below 15 the FP16
+# aggregate error is 9.4e-4 (90th percentile 7.2e-4).
+C4_MAX_CELL_RE = 15.0
+
+
+def c4_gate(specs, max_cell_re: float = C4_MAX_CELL_RE):
+    """Splits the batching workload at the stability bound: (admitted
+    specs, number rejected).  The rule is the library's (lbm_cell_reynolds,
+    the same one lbm_batch_add applies under lbm_batch_set_stability_limit),
+    so the admitted set is what a batch with that limit accepts."""
+    kept = [s for s in specs if capi.cell_reynolds(s[0]) <= max_cell_re]
+    return kept, len(specs) - len(kept)
+
+
 def with_storage(cfg: capi.Config, storage: int) -> capi.Config:
     c = capi.Config()
     C_fields = [f[0] for f in capi.Config._fields_]

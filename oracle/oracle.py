@@ -13,7 +13,10 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "liblbm_oracle.so")
+# LBM_ORACLE_LIB selects another build of the same sources, e.g. the
+# sanitizer build of `make -C oracle sanitize` (oracle/lib_san/)
+LIB_PATH = os.environ.get("LBM_ORACLE_LIB") or os.path.join(_HERE, "lib",
+                                                            "liblbm_oracle.so")
 
 FLAG_INTERNAL = 1
 FLAG_REFLECTED = 2

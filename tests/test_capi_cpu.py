@@ -83,3 +83,71 @@ def test_random_spheres_bit_exact_vs_oracle(dims, seed, px):
     assert (sp1 == sp2).all()
     assert (s1 == s2).all()
     assert s1.sum() >= 0.2 * np.prod(dims)
+
+
+def test_cell_reynolds_rule_without_gpu_work():
+    # U / nu with nu = (tau - 1/2) / 3: a lid at 0.1 and tau 0.51 gives 30
+    cav = capi.make_config(64, 48, 1, lattice=capi.D2Q9, tau=0.51,
+                           lid_face=3, lid_u=(0.1, 0, 0))
+    assert capi.cell_reynolds(cav) == pytest.approx(30.0, rel=1e-6)
+    # a body force F across walls W apart drives a Poiseuille channel at
+    # F W^2 / (8 nu): the force 8 nu U / W^2 of the batching workload
+    # gives U / nu back
+    tau, W, U = 0.8, 40, 0.05
+    nu = (tau - 0.5) / 3
+    ch = capi.make_config(200, W, 1, lattice=capi.D2Q9, tau=tau,
+                          periodic=(1, 0, 0), force=(8 * nu * U / W ** 2, 0, 0))
+    assert capi.cell_reynolds(ch) == pytest.approx(U / nu, rel=1e-6)
+    # 3-D channel: the narrowest walled extent (z = 20) sets W
+    ch3 = capi.make_config(64, 40, 20, lattice=capi.D3Q19, tau=tau,
+                           periodic=(1, 0, 0), force=(1e-6, 0, 0))
+    assert capi.cell_reynolds(ch3) == pytest.approx(
+        1e-6 * 20 ** 2 / (8 * nu) / nu, rel=1e-5)
+    # no walls: the largest extent
+    per = capi.make_config(64, 40, 20, lattice=capi.D3Q19, tau=tau,
+                           periodic=(1, 1, 1), force=(1e-6, 0, 0))
+    assert capi.cell_reynolds(per) == pytest.approx(
+        1e-6 * 64 ** 2 / (8 * nu) / nu, rel=1e-5)
+    # no drive: 0; invalid configuration: an error
+    still = capi.make_config(16, 16, 16, tau=0.6)
+    assert capi.cell_reynolds(still) == 0.0
+    with pytest.raises(capi.LbmError):
+        capi.cell_reynolds(capi.make_config(16, 16, 16, tau=0.5))
+    with pytest.raises(capi.LbmError):
+        capi.cell_reynolds(capi.make_config(4, 4, 4, lattice=27,
+                                            mode=capi.MODE_REFERENCE))
+
+
+def _c_layout(struct, fields, tmp_path):
+    """offsetof of each field and sizeof of a struct of include/lbm.h, as
+    the C compiler lays it out."""
+    import os
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "layout.c"
+    body = "".join(f'  printf("%zu\\n", offsetof({struct}, {f}));\n'
+                   for f in fields)
+    src.write_text("#include <stddef.h>\n#include <stdio.h>\n"
+                   '#include "lbm.h"\nint main(void) {\n' + body +
+                   f'  printf("%zu\\n", sizeof({struct}));\n  return 0;\n}}\n')
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(capi.ROOT, "include"), str(src),
+                    "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True,
+                         text=True).stdout.split()
+    return [int(v) for v in out]
+
+
+@pytest.mark.parametrize("cls,cname", [(capi.Config, "lbm_config"),
+                                       (capi.Request, "lbm_request"),
+                                       (capi.ServeParams, "lbm_serve_params"),
+                                       (capi.ServeStats, "lbm_serve_stats")])
+def test_struct_mirrors_match_header(cls, cname, tmp_path):
+    # the ctypes mirrors have the header's fields, offsets and size
+    fields = [f[0] for f in cls._fields_]
+    got = _c_layout(cname, fields, tmp_path)
+    assert got[:-1] == [getattr(cls, f).offset for f in fields]
+    assert got[-1] == C.sizeof(cls)

@@ -63,3 +63,54 @@ def test_cli_force_matches_oracle_main_loop():
     err = np.abs(F - Fo64).max() / np.abs(Fo64).max()
     say(f"  CLI {F} oracle exact {Fo64} rel {err:.2e}")
     assert err <= 1e-5
+
+
+def read_record_file(path, dims):
+    """The record file of Lattice::writeRecords: magic "LBMPTS1\\0", the
+    three extents (int32), then one 144-byte LatticePoint per node in index
+    order (the retrieve_data stream, Lattice.cpp:87-113)."""
+    from latticeboltzmanncuda_amd import capi
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"LBMPTS1\0"
+    assert list(np.frombuffer(raw[8:20], np.int32)) == list(dims)
+    pts = np.frombuffer(raw[20:], capi.POINT_DTYPE)
+    assert len(pts) == int(np.prod(dims))
+    return pts
+
+
+def test_cli_record_files_match_oracle(tmp_path):
+    # P1-style run (cube_angled at spacing 0.05 in 32^3, plane injection,
+    # force on): the records written after k steps hold exactly the
+    # oracle's populations after k steps, the node coordinates of the index
+    # order and the model's flags; a second run that starts from that file
+    # and takes m more steps writes exactly the oracle's state after k + m
+    k, m, tau, dims, spacing = 30, 20, 0.6, (32, 32, 32), 0.05
+    say(f"case: CLI record files, {k} + {m} steps vs oracle")
+    first, second = tmp_path / "k.rec", tmp_path / "km.rec"
+    run_cli("--quiet", "--write-records", str(first), steps=k, tau=tau)
+    run_cli("--quiet", "--read-records", str(first), "--write-records",
+            str(second), steps=m, tau=tau)
+    mesh = O.Mesh(os.path.join(DATA, "cube_angled.obj"))
+    ok, internal = mesh.insert(dims, spacing)
+    assert ok
+    flags, recs = mesh.preprocess(dims, spacing, internal * O.FLAG_INTERNAL)
+    n = int(np.prod(dims))
+    flow = np.full(27, 0.001, np.float32)
+    flow[6] = 0.002
+    f0 = np.tile(flow[:, None], (1, n)).reshape(-1).copy()
+    x, y, z = np.meshgrid(np.arange(dims[0]), np.arange(dims[1]),
+                          np.arange(dims[2]), indexing="ij")
+    for path, steps in ((first, k), (second, k + m)):
+        pts = read_record_file(path, dims)
+        fo = O.ref_run_main(dims, f0, flags, recs, tau, O.FLOW_PLANE, [0, 0],
+                            flow, np.zeros(3, np.float32), steps)
+        assert np.array_equal(pts["f"].T.reshape(-1), fo), steps
+        assert np.array_equal(pts["x"], x.reshape(-1))
+        assert np.array_equal(pts["y"], y.reshape(-1))
+        assert np.array_equal(pts["z"], z.reshape(-1))
+        assert np.array_equal(pts["isInternal"],
+                              (flags & O.FLAG_INTERNAL).astype(bool))
+        assert np.array_equal(pts["isReflected"],
+                              (flags & O.FLAG_REFLECTED).astype(bool))
+        assert not pts["reflections"].any() and not pts["d_reflections"].any()
+        say(f"  {steps} steps: populations, coordinates and flags exact")

@@ -10,7 +10,10 @@ import pytest
 from oracle import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-BIN = os.path.join(ROOT, "latticeboltzmanncuda_amd", "bin")
+# LBM_FACADE_BIN selects another build of the suites, e.g. the sanitizer
+# build (latticeboltzmanncuda_amd/bin_san, tests/run_sanitized.sh)
+BIN = os.environ.get("LBM_FACADE_BIN") or os.path.join(
+    ROOT, "latticeboltzmanncuda_amd", "bin")
 DATA = os.path.join(ROOT, "tests", "golden", "dataFiles")
 
 

@@ -127,6 +127,69 @@ def test_collision_mass_and_split_momentum():
     assert np.allclose(out2, 0.5 * (fr + out), rtol=1e-5, atol=1e-10)
 
 
+def _one_node(values):
+    f = np.zeros(27, np.float32)
+    for i, v in values.items():
+        f[i] = v
+    return f
+
+
+@pytest.mark.parametrize("values,rho,u", [
+    # f5 = 2, e5 = (0, 1, 0), |e|^2 = 1
+    ({5: 2.0}, 2.0, (0.0, 1.0, 0.0)),
+    # f7 = 2, e7 = (-1, -1, 0), SPLIT 1/2: u = 2 (-1, -1, 0) / 2 / 2
+    ({7: 2.0}, 2.0, (-0.5, -0.5, 0.0)),
+    # f19 = 3, e19 = (-1, -1, -1), SPLIT 1/3
+    ({19: 3.0}, 3.0, (-1 / 3, -1 / 3, -1 / 3)),
+    # all three shells at once, worked by hand: rho = 2,
+    # sum f e SPLIT = 0.3 (1,0,0) + 0.2 (0,1,1)/2 + 0.6 (-1,1,1)/3
+    #              = (0.1, 0.3, 0.3)  (without SPLIT it would be
+    #                (-0.3, 0.8, 0.8))
+    ({0: 0.9, 6: 0.3, 14: 0.2, 22: 0.6}, 2.0, (0.05, 0.15, 0.15)),
+    # zero density: the velocity stays the undivided sum (density != 0
+    # branch of SimCalcFuncs.cu:115-120), here 0
+    ({}, 0.0, (0.0, 0.0, 0.0)),
+    # zero density with nonzero populations: the sum is not divided
+    ({1: 1.0, 6: -1.0}, 0.0, (-2.0, 0.0, 0.0)),
+])
+def test_reference_moments_hand_computed(values, rho, u):
+    # SimCalcFuncs.cu:102-120: rho = sum f_i, u = sum f_i e_i SPLIT_i / rho
+    r, v = O.ref_moments((1, 1, 1), _one_node(values))
+    assert abs(r[0] - rho) < 1e-6
+    assert np.allclose(v[:, 0], u, atol=1e-6), v[:, 0]
+
+
+def test_reference_moments_node_order():
+    # plane i of the population array holds direction i for every node in
+    # index order n = z + nz (y + ny x): distinct nodes keep their values
+    dims = (2, 3, 2)
+    n = 12
+    f = np.zeros(27 * n, np.float32)
+    for k in range(n):
+        f[6 * n + k] = 0.1 * (k + 1)      # e6 = (1, 0, 0)
+        f[0 * n + k] = 1.0 - 0.1 * (k + 1)
+    r, v = O.ref_moments(dims, f)
+    assert np.allclose(r, 1.0, atol=1e-6)
+    assert np.allclose(v[0], 0.1 * np.arange(1, n + 1), atol=1e-6)
+    assert np.allclose(v[1:], 0.0)
+
+
+def test_reference_moments_match_collision_momentum():
+    # With tau = 1 the collision (pinned by CalculationsTest.cpp:47-105)
+    # returns feq(rho, u) whose first moment is rho u exactly (D3Q27
+    # isotropy), so the momentum after collision must equal rho * u from the
+    # moments readback on the same state.
+    rng = np.random.default_rng(11)
+    dims = (3, 2, 4)
+    n = 24
+    f = (0.01 + 0.02 * rng.random(27 * n)).astype(np.float32)
+    r, v = O.ref_moments(dims, f)
+    out = O.ref_collide(dims, f, 1.0).reshape(27, n).astype(np.float64)
+    mom = E27.T.astype(np.float64) @ out
+    assert np.allclose(out.sum(0), r, rtol=2e-6)
+    assert np.allclose(mom, r.astype(np.float64) * v, rtol=1e-4, atol=1e-8)
+
+
 def test_collision_rest_state_fixed_point():
     w = np.array([8 / 27] + [2 / 27] * 6 + [1 / 54] * 12 + [1 / 216] * 8)
     f = np.repeat((w * 1.3).astype(np.float32), 8)

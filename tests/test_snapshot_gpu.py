@@ -107,6 +107,83 @@ def test_snapshot_rejects_other_lattices(tmp_path):
             c.load(path)
 
 
+# Snapshot header (lbm_snapshot.hip SnapHeader, x86-64): magic 8 B, version
+# and header size (u32 each), lbm_config (72 B) at 16, steps at 88, then the
+# section sizes pop / mask / cls / flag (u64) at 96..127 and nrec (i32) at
+# 128.
+HDR_POP, HDR_NREC = 96, 128
+
+
+def library_tau(c):
+    """tau as the library holds it (lbm_get_config), not the Python copy."""
+    import ctypes
+    cfg = capi.Config()
+    capi.check(c.lib.lbm_get_config(c.h, ctypes.byref(cfg)), "lbm_get_config")
+    return cfg.tau
+
+
+def test_snapshot_rejects_corrupt_headers_without_changing_context(tmp_path):
+    say("case: snapshot with corrupt record count / flags")
+    dims, spacing = (16, 16, 16), 0.08
+    mesh = O.Mesh(os.path.join(DATA, "cube_angled.obj"))
+    ok, internal = mesh.insert(dims, spacing)
+    assert ok
+    flags, recs = mesh.preprocess(dims, spacing, internal * O.FLAG_INTERNAL)
+    assert len(recs[0]) > 0
+    cfg = capi.make_config(*dims, lattice=27, mode=capi.MODE_REFERENCE,
+                           tau=0.6)
+    path = tmp_path / "ref.bin"
+    with capi.Context(cfg) as a:
+        a.set_populations(np.full(27 * a.n, 0.001, np.float32))
+        a.upload_flags(flags)
+        a.upload_reflections(*recs)
+        a.step_reference(3, with_force=True)
+        a.save(path)
+    good = bytearray(open(path, "rb").read())
+    assert int.from_bytes(good[HDR_NREC:HDR_NREC + 4], "little") == len(recs[0])
+    hdr = int.from_bytes(good[12:16], "little")
+    pop = int.from_bytes(good[HDR_POP:HDR_POP + 8], "little")
+
+    def patched(fn):
+        b = bytearray(good)
+        fn(b)
+        p = tmp_path / "bad.bin"
+        p.write_bytes(bytes(b))
+        return p
+
+    bad = {
+        # a record count near 2^31 (would size ~700 GB of host vectors)
+        "huge nrec": patched(lambda b: b.__setitem__(
+            slice(HDR_NREC, HDR_NREC + 4), (2 ** 31 - 1).to_bytes(4, "little"))),
+        # one record more than the file holds
+        "nrec + 1": patched(lambda b: b.__setitem__(
+            slice(HDR_NREC, HDR_NREC + 4),
+            (len(recs[0]) + 1).to_bytes(4, "little"))),
+        # the first record's node flagged reflecting but not internal
+        "flags": patched(lambda b: b.__setitem__(
+            hdr + pop + int(recs[0][0]), capi.FLAG_REFLECTED)),
+        # a trailing byte
+        "size": patched(lambda b: b.extend(b"\0")),
+    }
+    other = capi.make_config(*dims, lattice=27, mode=capi.MODE_REFERENCE,
+                             tau=1.1)
+    with capi.Context(other) as c:
+        f0 = np.full(27 * c.n, 0.002, np.float32)
+        c.set_populations(f0)
+        for name, p in bad.items():
+            with pytest.raises(capi.LbmError) as e:
+                c.load(p)
+            say(f"  {name}: {e.value}")
+            assert e.value.code == 1  # LBM_ERR_INVALID
+            # nothing changed: parameters, populations, flags
+            assert abs(library_tau(c) - 1.1) < 1e-7
+            assert np.array_equal(c.populations(), f0)
+            assert not c.flags().any()
+        c.load(path)  # the intact file still loads
+        assert abs(library_tau(c) - 0.6) < 1e-7
+        assert np.array_equal(c.flags(), flags)
+
+
 def test_batch_member_takes_snapshot_with_other_moving_wall(tmp_path):
     # a batch member loads snapshots whose moving wall differs from its own
     # (lattices with and without one share the batch kernel): after the

@@ -236,6 +236,16 @@ int lbm_moments(lbm_ctx* ctx, float* rho, float* u);
 /* FP64 totals over non-solid nodes: out[0] = mass, out[1..3] = momentum
  * sum f e. */
 int lbm_conservation(lbm_ctx* ctx, double out[4]);
+/* No counterpart in the reference; for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  lbm_moments without any allocation:
+ * the same kernels and results, staged in a fixed 64 MiB device array of
+ * the library (one per GPU, held until the process ends) instead of a
+ * buffer it allocates.  Lattices needing more than that -- 16 bytes per
+ * node for rho and u (4,194,304 nodes), 12 for u alone, 4 for rho alone --
+ * are refused with LBM_ERR_INVALID.  Calls on one GPU share the array and
+ * are serialised by a lock held until the readback has completed. */
+int lbm_moments_noalloc(lbm_ctx* ctx, float* rho, float* u);
 
 /* ------------------------------------------------------------ test helper */
 /* Replaces RunCudaTestFunctions::run_test_coordinates

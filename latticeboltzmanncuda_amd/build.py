@@ -46,7 +46,7 @@ HIP_FLAGS = [
 HIP_SOURCES = ["lbm_capi.hip", "k_standard.hip", "k_reference.hip",
                "k_common.hip", "lbm_batch.hip", "k_batch.hip",
                "lbm_serve.hip", "k_voxel.hip", "lbm_snapshot.hip",
-               "lbm_exports.hip"]
+               "lbm_exports.hip", "lbm_noalloc.hip"]
 HOST_SOURCES = ["Constants.cpp", "Geometry.cpp", "Utils.cpp",
                 "FlowCriterion.cpp", "Model.cpp", "Lattice.cpp"]
 CXX = os.environ.get("CXX", "g++")

@@ -168,6 +168,7 @@ _SIGNATURES = {
     "lbm_time_steps_reference": (C.c_int, [CTX, C.c_int, C.c_int,
                                            C.POINTER(C.c_float)]),
     "lbm_moments": (C.c_int, [CTX, P, P]),
+    "lbm_moments_noalloc": (C.c_int, [CTX, P, P]),
     "lbm_conservation": (C.c_int, [CTX, P]),
     "lbm_test_coordinates": (C.c_int, [CTX, C.POINTER(C.c_int)]),
     "lbm_batch_create": (C.c_int, [C.c_int, i64, P, C.POINTER(P)]),

@@ -48,6 +48,7 @@
   X(int, lbm_time_steps, (lbm_ctx* a, int b, float* c), (a, b, c))                             \
   X(int, lbm_time_steps_reference, (lbm_ctx* a, int b, int c, float* d), (a, b, c, d))         \
   X(int, lbm_moments, (lbm_ctx* a, float* b, float* c), (a, b, c))                             \
+  X(int, lbm_moments_noalloc, (lbm_ctx* a, float* b, float* c), (a, b, c))                     \
   X(int, lbm_conservation, (lbm_ctx* a, double b[4]), (a, b))                                  \
   X(int, lbm_test_coordinates, (lbm_ctx* a, int* b), (a, b))                                   \
   X(int, lbm_measure_copy_bandwidth, (int64_t a, int b, double* c), (a, b, c))                 \

@@ -52,7 +52,9 @@ def parse():
                     help="skip the extra sections")
     ap.add_argument("--sections",
                     default="precision,lattice2d,batched,serving,reference",
-                    help="comma list of the extra sections to run")
+                    help="comma list of the extra sections to run "
+                         "(also: c4kernel, the batch kernel alone for "
+                         "profilers)")
     ap.add_argument("--c3-steps", type=int, default=2000)
     ap.add_argument("--l2d-steps", type=int, default=1000)
     ap.add_argument("--ref-steps", type=int, default=1000)
@@ -130,6 +132,55 @@ def batched_section(steps, launch=None, chunk=100):
     out["f16"]["error_vs_f64"] = {
         k: (round(v, 8) if isinstance(v, float) else v)
         for k, v in err.items()}
+    return out
+
+
+def c4kernel_section(steps, launch=None):
+    """The batch kernel alone, for profilers: the admitted lattices of the
+    batching workload in FP16 stepped with plain launches (no graphs, so
+    every dispatch is visible to a kernel tracer), the 4096^2 D2Q9 FP16
+    channel with one node per thread as the single-lattice control, and a
+    256 MiB copy of known size for calibrating byte counters."""
+    from latticeboltzmanncuda_amd import capi, workloads
+    out = {"steps": steps}
+    out["copy_256MiB_gbs"] = round(capi.measure_copy_bandwidth(256 << 20, 5), 1)
+    specs, rejected = workloads.c4_gate(
+        workloads.c4_specs(512, 4, storage=capi.STORE_F16)[0])
+    with capi.Batch(capi.STORE_F16) as b:
+        for cfg, solid, _ in specs:
+            b.add(cfg, solid)
+        if launch:
+            b.set_launch(*launch)
+            blk, grd = launch
+        else:
+            blk, grd = b.autotune(steps=10)
+        _, nodes, _ = b.info()
+        b.step(10, 0)
+        b.synchronize()
+        us = b.time_steps(steps, 0) / steps * 1e3
+    out["batch"] = {"lattices": len(specs), "rejected_unstable": rejected,
+                    "nodes": nodes, "block": blk, "grid": grd,
+                    "us_per_step": round(us, 2),
+                    "glups": round(nodes / us / 1e3, 1),
+                    "hbm_frac_36B": round(nodes * 36 / (us * 1e-6) / 8e12, 4)}
+    n = 4096
+    sq = workloads.random_squares(n, n, 0.1, np.random.default_rng(7))
+    os.environ["LBM_GROUP_STEP"] = "0"
+    try:
+        cfg = capi.make_config(n, n, 1, lattice=capi.D2Q9,
+                               storage=capi.STORE_F16, tau=0.8,
+                               periodic=(1, 0, 0), force=(1e-6, 0, 0))
+        with capi.Context(cfg) as c:
+            c.upload_solid(sq)
+            c.init_equilibrium()
+            c.step(10)
+            c.synchronize()
+            us = c.time_steps(steps) / steps * 1e3
+    finally:
+        os.environ.pop("LBM_GROUP_STEP", None)
+    out["single_4096_channel_one_node"] = {
+        "us_per_step": round(us, 2), "glups": round(n * n / us / 1e3, 1),
+        "hbm_frac_36B": round(n * n * 36 / (us * 1e-6) / 8e12, 4)}
     return out
 
 
@@ -458,11 +509,14 @@ def main():
     if "lattice2d" in sections:
         log("lattice2d")
         out["lattice2d"] = lattice2d_section(args.l2d_steps)
+    launch = (tuple(int(v) for v in args.c4_launch.split(","))
+              if args.c4_launch else None)
     if "batched" in sections:
         log("batched")
-        launch = (tuple(int(v) for v in args.c4_launch.split(","))
-                  if args.c4_launch else None)
         out["batched"] = batched_section(args.c4_steps, launch, args.c4_chunk)
+    if "c4kernel" in sections:
+        log("c4kernel")
+        out["c4kernel"] = c4kernel_section(args.c4_steps, launch)
     if "reference" in sections:
         log("reference mode")
         out["reference_mode"] = reference_mode_section(args.ref_steps)

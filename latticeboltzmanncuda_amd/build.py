@@ -86,8 +86,11 @@ def build_hip(jobs: int | None = None) -> str:
         objs = list(ex.map(_compile_hip, HIP_SOURCES))
     out = os.path.join(LIBDIR, "liblbm_hip.so")
     tmp = out + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o",
-          tmp])
+    # roctx ranges (rocprofiler-sdk) mark step and serving chunks for
+    # rocprofv3 --marker-trace
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs,
+          "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib",
+          "-o", tmp])
     os.replace(tmp, out)
     return out
 

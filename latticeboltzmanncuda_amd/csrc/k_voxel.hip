@@ -299,6 +299,7 @@ int lbm_build_reflections(lbm_ctx* c, const float* vertices, int32_t nv,
                           int32_t max_records, int32_t* node, uint32_t* dirs,
                           float* weights, int32_t* count) {
   if (!c) return fail(LBM_ERR_INVALID, "null context");
+  TraceRange tr("lbm_build_reflections");
   if (c->cfg.mode != LBM_MODE_REFERENCE)
     return fail(LBM_ERR_STATE, "lbm_build_reflections needs a reference-mode lattice");
   if (!count || nv < 0 || nf < 0 || max_records < 0 || (nf > 0 && (!vertices || !faces)) ||
@@ -382,6 +383,7 @@ int lbm_voxelize_mesh(lbm_ctx* c, const float* vertices, int32_t nv,
                       const int32_t* faces, int32_t nf, float spacing,
                       uint8_t* inside, int64_t* count) {
   if (!c) return fail(LBM_ERR_INVALID, "null context");
+  TraceRange tr("lbm_voxelize_mesh");
   if (!inside || nv < 0 || nf < 0 || (nf > 0 && (!vertices || !faces)))
     return fail(LBM_ERR_INVALID, "bad mesh arguments");
   for (int64_t k = 0; k < 3 * (int64_t)nf; ++k)

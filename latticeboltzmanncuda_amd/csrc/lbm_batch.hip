@@ -459,6 +459,7 @@ int lbm_batch_get_launch(const lbm_batch* b, int* block, int* grid) {
 int lbm_batch_step(lbm_batch* b, int n) {
   if (int rc = check_batch(b)) return rc;
   if (n < 0) return fail(LBM_ERR_INVALID, "negative step count");
+  TraceRange tr("lbm_batch_step");
   int rc = sync_tables(b);
   if (rc || n == 0 || b->ntiles == 0) return rc;
   if ((rc = launch_steps(b, n))) return rc;
@@ -470,6 +471,7 @@ int lbm_batch_step_graph(lbm_batch* b, int n, int chunk) {
   if (int rc = check_batch(b)) return rc;
   if (n < 0 || chunk < 2 || (chunk & 1))
     return fail(LBM_ERR_INVALID, "chunk must be an even step count >= 2");
+  TraceRange tr("lbm_batch_step_graph");
   int rc = sync_tables(b);
   if (rc || n == 0 || b->ntiles == 0) return rc;
   const int full = n / chunk;
@@ -510,6 +512,7 @@ int lbm_batch_autotune(lbm_batch* b, int steps, int use_cache, int* block_out,
                        int* grid_out) {
   if (int rc = check_batch(b)) return rc;
   if (steps < 2) return fail(LBM_ERR_INVALID, "autotune needs >= 2 steps");
+  TraceRange tr("lbm_batch_autotune");
   steps += steps & 1;  // even: members end where they started
   const auto key = tune_key(b);
   if (use_cache) {
@@ -567,6 +570,7 @@ int lbm_batch_residuals(lbm_batch* b, double* out, int32_t nslots) {
   if (int rc = check_batch(b)) return rc;
   if (!out || nslots < (int32_t)b->slots.size())
     return fail(LBM_ERR_INVALID, "output must hold one value per slot");
+  TraceRange tr("lbm_batch_residuals");
   int rc = sync_tables(b);
   if (rc) return rc;
   for (int32_t s = 0; s < nslots; ++s) out[s] = -1.0;

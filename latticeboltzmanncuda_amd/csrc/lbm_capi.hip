@@ -941,12 +941,14 @@ int lbm_get_force(lbm_ctx* c, float out[3]) {
 int lbm_step(lbm_ctx* c, int n) {
   if (int rc = check_ctx(c)) return rc;
   if (n < 0) return fail(LBM_ERR_INVALID, "negative step count");
+  TraceRange tr("lbm_step");
   return c->cfg.mode == LBM_MODE_STANDARD ? step_standard(c, n)
                                           : step_reference_fused(c, n);
 }
 
 int lbm_step_each(lbm_ctx* const* ctxs, int32_t count, int n) {
   if (!ctxs || count < 0 || n < 0) return fail(LBM_ERR_INVALID, "bad arguments");
+  TraceRange tr("lbm_step_each");
   for (int32_t k = 0; k < count; ++k) {
     if (int rc = check_ctx(ctxs[k])) return rc;
     if (ctxs[k]->cfg.mode != LBM_MODE_STANDARD)
@@ -962,6 +964,7 @@ int lbm_step_reference(lbm_ctx* c, int n, int with_force) {
   if (int rc = check_ctx(c)) return rc;
   if (int rc = need_mode(c, LBM_MODE_REFERENCE, "lbm_step_reference")) return rc;
   if (n < 0) return fail(LBM_ERR_INVALID, "negative step count");
+  TraceRange tr("lbm_step_reference");
   return step_reference_full(c, n, with_force);
 }
 

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdint>
 #include <string>
@@ -11,6 +12,17 @@
 #include "../../include/lbm.h"
 
 namespace lbm {
+
+// A roctx range over a host-side scope -- step calls, hipGraph chunks,
+// serving chunks, geometry builds -- shown by `rocprofv3 --marker-trace`
+// on the timeline beside the kernels it launched (a no-op call when no
+// profiler is attached).
+struct TraceRange {
+  explicit TraceRange(const char* name) { roctxRangePushA(name); }
+  ~TraceRange() { roctxRangePop(); }
+  TraceRange(const TraceRange&) = delete;
+  TraceRange& operator=(const TraceRange&) = delete;
+};
 
 // Division by a runtime constant d for dividends below 2^31:
 // q = (umulhi(n, mul) + n) >> shift  (Granlund-Montgomery round-up method).

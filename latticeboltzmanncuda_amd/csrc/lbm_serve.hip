@@ -264,7 +264,9 @@ int lbm_serve_run(const lbm_serve_params* p, lbm_request* reqs, int32_t nreq,
       }
       continue;
     }
-    // one chunk for everybody resident
+    // one chunk for everybody resident (the range also covers the
+    // residuals and retirements that follow it)
+    TraceRange chunk_range("lbm_serve chunk");
     int64_t nodes = 0;
     lbm_batch_info(b, nullptr, &nodes, nullptr);
     if (p->engine == LBM_ENGINE_PER_LATTICE) {

@@ -103,6 +103,10 @@ class Lattice {
 
   dim3 getDimensions() { return dim3(m_xResolution, m_yResolution, m_zResolution); }
   LatticeHandle getDeviceHandle() { return m_ctx; }
+  // The reference's name for the same accessor (Lattice.h:51), so its
+  // callers compile unchanged; it returns the opaque handle, not a
+  // pitched pointer.
+  LatticeHandle getCudaDataPointer() { return m_ctx; }
 
   void simulateStreaming();
   void simulateCollision();
@@ -186,3 +190,9 @@ void run_test_streaming(dim3 threads, dim3 blocks, LatticeData lattice,
                         LatticeData templattice);
 void run_prime_points(dim3 threads, dim3 blocks, LatticeData lattice);
 }  // namespace RunHipTestFunctions
+
+// The reference's namespace names (SimCalcFuncs.cuh:13, CudaTestHelper.cuh:6)
+// for the same functions, so the reference's callers and tests compile
+// unchanged.  Nothing CUDA is behind them: they are the HIP functions above.
+namespace RunCudaFunctions = RunHipFunctions;
+namespace RunCudaTestFunctions = RunHipTestFunctions;

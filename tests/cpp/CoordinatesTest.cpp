@@ -53,7 +53,9 @@ TEST(CoordinatesTest, TestCoordinatesOnCopy) {
   auto pts = grid_points(100);
   lat.load_data(pts.get());
   bool result = true;
-  RunHipTestFunctions::run_test_coordinates(threads, blocks, lat.getDeviceHandle(), 100, &result);
+  // the reference's spelling of the call (CoordinatesTest.cpp:130)
+  RunCudaTestFunctions::run_test_coordinates(threads, blocks, lat.getCudaDataPointer(), 100,
+                                             &result);
   EXPECT_TRUE(result);
 }
 

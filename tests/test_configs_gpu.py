@@ -98,6 +98,39 @@ def test_c2_headline_arithmetic_1000_steps_vs_oracle():
     assert d_gpu <= 4.0 * d_ora
 
 
+def test_c2_full_size_fp32_vs_fp64_1000_steps():
+    # the headline lattice at its full 256x128x128 for 1000 steps: FP32
+    # storage against FP64 storage on the GPU (the FP64 step matches the
+    # FP64 oracle to 1e-12 on smaller lattices, test_parity_gpu), and the
+    # FP64 run conserves the fluid mass
+    dims, tau, steps, F = (256, 128, 128), 0.8, 1000, (1e-6, 0, 0)
+    say(f"case: C2 full size {dims} x {steps}, FP32 vs FP64 storage")
+    solid, _ = capi.random_spheres(dims, 2, 0.2, 4.0, 8.0, True)
+    out = {}
+    for st in (capi.STORE_F32, capi.STORE_F64):
+        cfg = capi.make_config(*dims, lattice=19, storage=st, tau=tau,
+                               periodic=(1, 0, 0), force=F)
+        with capi.Context(cfg) as c:
+            c.upload_solid(solid)
+            c.init_equilibrium()
+            m0 = c.conservation()[0]
+            c.step(steps)
+            m1 = c.conservation()[0]
+            rho, u = c.moments()
+            f = c.populations(np.float64 if st == capi.STORE_F64
+                              else np.float32)
+        out[st] = (f, rho, u, m0, m1)
+    f32, r32, u32, _, _ = out[capi.STORE_F32]
+    f64, r64, u64, m0, m1 = out[capi.STORE_F64]
+    ef, er, eu = rel_err(f32, f64), rel_err(r32, r64), vel_err(u32, u64)
+    say(f"  f {ef:.2e} rho {er:.2e} u {eu:.2e} (max |u| "
+        f"{np.abs(u64).max():.2e}); FP64 mass drift {abs(m1 - m0) / m0:.2e}")
+    assert ef <= TOL and er <= TOL and eu <= TOL
+    # FP64 rounding only (1e-12 over 200 steps in test_parity_gpu)
+    assert abs(m1 - m0) <= 1e-11 * m0
+    assert u64[0][solid == 0].mean() > 0
+
+
 # ------------------------------------------------------------------ C3
 def test_c3_taylor_green_256_all_storages():
     # full size: 256^3, u0 0.05, tau 0.55, 2000 steps

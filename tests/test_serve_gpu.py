@@ -111,6 +111,53 @@ def test_serve_node_cap_queues_without_changing_results():
         assert np.array_equal(a["u"], b["u"])
 
 
+def test_serve_runs_requests_to_steady_state():
+    # FP32 body-force channels short enough to reach a 1e-6 residual well
+    # inside their step limit (spin-up time ~ ny^2 / (pi^2 nu)): each
+    # retires converged (status 0) at the first chunk below tol, with the
+    # bytes of a plain context stepped as far, and its profile is the
+    # Poiseuille parabola of the half-way bounce-back walls
+    say("case: serve FP32 channels to a 1e-6 residual")
+    reqs = []
+    for k, (ny, tau) in enumerate(((16, 1.0), (24, 0.9), (20, 1.2))):
+        nu = (tau - 0.5) / 3.0
+        cfg = capi.make_config(8, ny, 1, lattice=9, tau=tau,
+                               periodic=(1, 0, 0),
+                               force=(8.0 * nu * 0.01 / ny ** 2, 0, 0))
+        reqs.append({"cfg": cfg, "solid": None, "arrival_s": 0.001 * k,
+                     "max_steps": 40000, "tol": 1e-6})
+    stats, res = capi.serve(reqs, storage=capi.STORE_F32, chunk=100,
+                            keep_results=True)
+    say(f"  steps {[r['steps'] for r in res]} residuals "
+        f"{[r['residual'] for r in res]}")
+    for r, out in zip(reqs, res):
+        assert out["status"] == capi.SERVE_CONVERGED
+        assert 0 <= out["residual"] < 1e-6
+        assert out["steps"] < r["max_steps"]
+        assert np.array_equal(out["u"], plain_u(r, out["steps"]))
+        cfg = r["cfg"]
+        ny, nu = cfg.ny, (cfg.tau - 0.5) / 3.0
+        y = np.arange(ny) + 0.5  # walls half a link outside the domain
+        exact = cfg.force[0] / (2.0 * nu) * y * (ny - y)
+        ux = out["u"][0].reshape(cfg.nx, ny).mean(0)
+        # half-way bounce-back puts the wall exactly half a link out only
+        # at tau = 1/2 + sqrt(3/16); elsewhere the wall slips by
+        # O((tau - 1/2)^2 / ny^2) of the centre-line speed (the oracle's
+        # FP32 run: 1.3e-3, 2.5e-4, 4.1e-3 for these three)
+        err = np.abs(ux - exact).max() / exact.max()
+        say(f"  ny {ny} tau {cfg.tau}: steady profile vs parabola {err:.2e}")
+        assert err < 1e-2
+    assert stats["completed"] == len(reqs)
+
+
+def plain_u(r, steps):
+    with capi.Context(r["cfg"]) as c:
+        c.upload_solid(r["solid"])
+        c.init_equilibrium()
+        c.step(steps)
+        return c.moments()[1]
+
+
 def test_serve_fp16_accuracy_vs_fp64():
     say("case: serve FP16 storage vs FP64 storage")
     r16 = make_requests(storage=capi.STORE_F16, n=6, seed=9)

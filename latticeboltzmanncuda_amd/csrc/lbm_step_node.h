@@ -122,16 +122,52 @@ __device__ __forceinline__ void std_gather(const Grid& g,
 #pragma unroll
   for (int i = 0; i < Q; ++i) ld.raw[i] = __builtin_nondeterministic_value(ld.raw[i]);
   if (!ld.solid) {
+    using T = typename Store<S>::T;
+    if constexpr (sizeof(T) == 2) {
+      // 16-bit storage, in byte offsets (below 2^32: lbm_create keeps
+      // element indices below 2^31): element (m, i) lies at
+      //   2 pidx(m, i) = 2 m + (m / 64) (Q - 1) 128 + 128 i,
+      // one 24-bit multiply-add per source node.  The part of the offset
+      // that the pulled address (direction i of the source node) and the
+      // bounced one (direction INV[i] of the node itself) share,
+      // 128 min(i, INV[i]), is a constant of the load instruction; what is
+      // left of either is non-negative, so the 32-bit offset cannot wrap.
+      constexpr uint32_t kRow = (uint32_t)kTile * 2u;  // bytes of one direction's run
+      const uint32_t own_b = 2u * ld.own;
+      auto pull_b = [&](int i) {
+        const uint32_t src = n + ox[L::E[i][0] + 1] + oy[L::E[i][1] + 1] + oz[L::E[i][2] + 1];
+        return __umul24(src / kTile, (uint32_t)(Q - 1) * kRow) + 2u * src;
+      };
+      // a wavefront without a bounced link among its nodes (a third of
+      // them in the batching workload, more in large lattices) skips the
+      // per-link selection: the branch is uniform
+      if (!__any(ld.m != 0u)) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      const uint32_t src = n + ox[L::E[i][0] + 1] + oy[L::E[i][1] + 1] + oz[L::E[i][2] + 1];
-      // pidx(src, i) as one 24-bit multiply-add: lane | i * kTile < Q * kTile
-      const uint32_t from =
-          __umul24(src / kTile, (uint32_t)(Q * kTile)) + ((src % kTile) | ((uint32_t)i * kTile));
-      const uint32_t back = ld.own + (uint32_t)L::INV[i] * kTile;
-      // bounced link: read the node's own opposite population instead
-      const uint32_t sel = 0u - ((ld.m >> i) & 1u);
-      ld.raw[i] = gload(elem(A, from ^ ((from ^ back) & sel)));
+        for (int i = 0; i < Q; ++i)
+          ld.raw[i] = gload((const T*)((const char*)A + pull_b(i) + (uint32_t)i * kRow));
+      } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+          const uint32_t lo = (uint32_t)(i < L::INV[i] ? i : L::INV[i]) * kRow;
+          const uint32_t a = pull_b(i) + ((uint32_t)i * kRow - lo);
+          const uint32_t b = own_b + ((uint32_t)L::INV[i] * kRow - lo);
+          // bounced link: read the node's own opposite population instead
+          const uint32_t off = ((ld.m >> i) & 1u) ? b : a;
+          ld.raw[i] = gload((const T*)((const char*)A + off + lo));
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const uint32_t src = n + ox[L::E[i][0] + 1] + oy[L::E[i][1] + 1] + oz[L::E[i][2] + 1];
+        // pidx(src, i) as one 24-bit multiply-add: lane | i * kTile < Q * kTile
+        const uint32_t from =
+            __umul24(src / kTile, (uint32_t)(Q * kTile)) + ((src % kTile) | ((uint32_t)i * kTile));
+        const uint32_t back = ld.own + (uint32_t)L::INV[i] * kTile;
+        // bounced link: read the node's own opposite population instead
+        const uint32_t sel = 0u - ((ld.m >> i) & 1u);
+        ld.raw[i] = gload(elem(A, from ^ ((from ^ back) & sel)));
+      }
     }
   }
 }

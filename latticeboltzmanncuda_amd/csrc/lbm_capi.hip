@@ -89,10 +89,24 @@ void fill_coef(const StdParams& p, StepCoef<C>& k) {
   }
 }
 
+// cf in FP16 stored units: exact, as the scale is a power of two
+template <class L>
+void fill_coef_units(const StepCoef<float>& k, float scale, StepCoef<float>& u) {
+  u = k;
+  for (int a = 0; a < 3; ++a) u.half_force[a] = k.half_force[a] * scale;
+  for (int i = 0; i < L::Q; ++i) {
+    u.lid[i] = k.lid[i] * scale;
+    u.gw[i] = k.gw[i] * scale;
+    u.gf3[i] = k.gf3[i] * scale;
+    u.gf9[i] = k.gf9[i] * scale;
+  }
+}
+
 template <class L>
 void fill_coefs(StdParams& p) {
   fill_coef<L, float>(p, p.cf);
   fill_coef<L, double>(p, p.cd);
+  fill_coef_units<L>(p.cf, 1.0f / Store<LBM_STORE_F16>::kUnit, p.ch);
 }
 
 }  // namespace

@@ -78,8 +78,12 @@ struct StdParams {
   int lid_sign;    // +1 max face, -1 min face
   float lid_u[3];
   float wall_rho;
-  StepCoef<float> cf;   // FP32 compute (FP32 / FP16 / BF16 storage)
+  StepCoef<float> cf;   // FP32 compute (FP32 / BF16 storage)
   StepCoef<double> cd;  // FP64 compute (FP64 storage)
+  // FP32 compute in FP16 stored units (2^12 x real): cf with every term
+  // that is added to a stored value (half_force, lid, gw, gf3, gf9) scaled
+  // by 2^12; omega and force as in cf
+  StepCoef<float> ch;
 };
 
 struct FlowSpec {

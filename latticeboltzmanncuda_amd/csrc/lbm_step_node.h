@@ -55,15 +55,15 @@ __device__ __forceinline__ void decode_node(const Grid& g, uint32_t n,
   y = t - __umul24(x, g.ny);  // x < 2^24 and ny < 2^24 (lbm_create checks)
 }
 
-template <class C>
-__device__ __forceinline__ const StepCoef<C>& step_coef(const StdParams& p);
-template <>
-__device__ __forceinline__ const StepCoef<float>& step_coef<float>(const StdParams& p) {
-  return p.cf;
-}
-template <>
-__device__ __forceinline__ const StepCoef<double>& step_coef<double>(const StdParams& p) {
-  return p.cd;
+// The step constants of storage type S, in its stored units.
+template <int S>
+__device__ __forceinline__ const StepCoef<typename Store<S>::C>& step_coef(const StdParams& p) {
+  if constexpr (S == LBM_STORE_F64)
+    return p.cd;
+  else if constexpr (S == LBM_STORE_F16)
+    return p.ch;
+  else
+    return p.cf;
 }
 
 // 1 / r: IEEE division in FP64; in FP32 the hardware reciprocal refined by
@@ -125,14 +125,15 @@ __device__ __forceinline__ void std_gather(const Grid& g,
     using T = typename Store<S>::T;
     if constexpr (sizeof(T) == 2) {
       // 16-bit storage, in byte offsets (below 2^32: lbm_create keeps
-      // element indices below 2^31): element (m, i) lies at
+      // element indices below 2^31) through a buffer resource over the
+      // population array: element (m, i) lies at
       //   2 pidx(m, i) = 2 m + (m / 64) (Q - 1) 128 + 128 i,
       // one 24-bit multiply-add per source node.  The part of the offset
       // that the pulled address (direction i of the source node) and the
       // bounced one (direction INV[i] of the node itself) share,
-      // 128 min(i, INV[i]), is a constant of the load instruction; what is
-      // left of either is non-negative, so the 32-bit offset cannot wrap.
+      // 128 min(i, INV[i]), is a constant of the load instruction.
       constexpr uint32_t kRow = (uint32_t)kTile * 2u;  // bytes of one direction's run
+      const auto rsrc = buffer_rsrc(A, (g.n + kTile - 1u) / kTile * (uint32_t)Q * kRow);
       const uint32_t own_b = 2u * ld.own;
       auto pull_b = [&](int i) {
         const uint32_t src = n + ox[L::E[i][0] + 1] + oy[L::E[i][1] + 1] + oz[L::E[i][2] + 1];
@@ -143,8 +144,7 @@ __device__ __forceinline__ void std_gather(const Grid& g,
       // per-link selection: the branch is uniform
       if (!__any(ld.m != 0u)) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i)
-          ld.raw[i] = gload((const T*)((const char*)A + pull_b(i) + (uint32_t)i * kRow));
+        for (int i = 0; i < Q; ++i) ld.raw[i] = bload16<T>(rsrc, pull_b(i) + (uint32_t)i * kRow);
       } else {
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
@@ -153,7 +153,7 @@ __device__ __forceinline__ void std_gather(const Grid& g,
           const uint32_t b = own_b + ((uint32_t)L::INV[i] * kRow - lo);
           // bounced link: read the node's own opposite population instead
           const uint32_t off = ((ld.m >> i) & 1u) ? b : a;
-          ld.raw[i] = gload((const T*)((const char*)A + off + lo));
+          ld.raw[i] = bload16<T>(rsrc, off + lo);
         }
       }
     } else {
@@ -179,8 +179,9 @@ struct Forced {
 
 // BGK relaxation towards the equilibrium of (rho, u), plus the Guo source
 // term when FORCED; emit(Forced<FORCED>{}, i, out) receives the Q outputs
-// in the stored form (solid nodes: the rest state).  rs is rho (full
-// storage) or drho = rho - 1 (shifted storage).
+// in the stored form and stored units (solid nodes: the rest state).  gv
+// holds the inputs in stored units, k the constants in stored units; rs is
+// rho (full storage) or drho = rho - 1 (shifted storage), in real units.
 // Shifted storage forms feq - f = w rho t - w - g as
 //   w ((t - 1) + drho + drho (t - 1)) - g
 // from small terms only: no intermediate is near 1 or near w, so the
@@ -208,10 +209,11 @@ __device__ __forceinline__ void std_relax(const StepCoef<typename Store<S>::C>& 
     if (L::E[i][1]) eu = L::E[i][1] > 0 ? eu + uy : eu - uy;
     if (L::E[i][2]) eu = L::E[i][2] > 0 ? eu + uz : eu - uz;
     const C w = (C)L::W[i];
+    const C wu = w / St::kUnit;  // w in stored units (exact: kUnit is 2^k)
     const C t = fmadd(fmadd((C)4.5, eu, (C)3), eu, base);
     // feq - f in the stored form: w ((t-1) + drho + drho (t-1)) - g
     // (shifted, t here is t - 1), w rho t - f (full)
-    const C d = kSh ? fmadd(w, fmadd(rs, t, t + rs), -gv[i]) : (w * rs) * t - gv[i];
+    const C d = kSh ? fmadd(wu, fmadd(rs, t, t + rs), -gv[i]) : (w * rs) * t - gv[i];
     C out = fmadd(k.omega, d, gv[i]);
     if (FORCED) {
       // Guo source (1 - omega/2) w (3 (e.F - u.F) + 9 (e.u)(e.F)), added
@@ -226,8 +228,9 @@ __device__ __forceinline__ void std_relax(const StepCoef<typename Store<S>::C>& 
 }
 
 // The collision of one node from its Q pulled (or bounced) values in the
-// stored form, gv[i] (f_i, or f_i - w_i for shifted storage), its link
-// mask m and solid flag; emit receives the outputs (see std_relax).
+// stored form and stored units, gv[i] (f_i, or f_i - w_i for shifted
+// storage), its link mask m and solid flag; emit receives the outputs (see
+// std_relax).
 template <int Q, int S, bool LID, class Emit>
 __device__ __forceinline__ void std_collide_node(const StdParams& p,
                                                  typename Store<S>::C (&gv)[Q], uint32_t m,
@@ -240,29 +243,37 @@ __device__ __forceinline__ void std_collide_node(const StdParams& p,
   using St = Store<S>;
   using C = typename St::C;
   constexpr bool kSh = St::kShifted;
-  const StepCoef<C>& k = step_coef<C>(p);
-  // the moving-wall term of bounced links from beyond the moving face
-  const bool lidnode = LID && (m & LBM_MASK_LID);
+  const StepCoef<C>& k = step_coef<S>(p);
+  // the moving-wall term of bounced links from beyond the moving face:
+  // only nodes next to the moving face have such links, so a wavefront
+  // without one skips the selection (the branch is uniform)
+  if (LID) {
+    const bool lidnode = (m & LBM_MASK_LID) != 0u;
+    if (__any(lidnode)) {
 #pragma unroll
-  for (int i = 1; i < Q; ++i) {  // the rest direction never crosses a face
-    if (LID) {
-      const bool wall = lidnode && ((m >> i) & 1u);
-      gv[i] += wall ? k.lid[i] : (C)0;
+      for (int i = 1; i < Q; ++i) {  // the rest direction never crosses a face
+        const bool wall = lidnode && ((m >> i) & 1u);
+        gv[i] += wall ? k.lid[i] : (C)0;
+      }
     }
   }
 
   // moments; with shifted storage sum w_i = 1 and sum w_i e_i = 0, so the
   // sum of the stored values is rho itself (full storage) or the density
   // deviation drho = rho - 1 (shifted storage)
-  C rs = 0, jx = 0, jy = 0, jz = 0;
+  // (the sum starts from the rest direction's value, not from 0 + it: the
+  // two differ only for all inputs -0, whose outputs are the same)
+  C rsu = gv[0], jx = 0, jy = 0, jz = 0;  // stored units
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    rs += gv[i];
+    if (i) rsu += gv[i];
     if (L::E[i][0]) jx += L::E[i][0] > 0 ? gv[i] : -gv[i];
     if (L::E[i][1]) jy += L::E[i][1] > 0 ? gv[i] : -gv[i];
     if (L::E[i][2]) jz += L::E[i][2] > 0 ? gv[i] : -gv[i];
   }
-  const C ir = recip(kSh ? (C)1 + rs : rs);
+  const C rs = rsu * St::kUnit;
+  // 1 / rho, times kUnit: j (stored units) times it is u
+  const C ir = recip(kSh ? (C)1 + rs : rs) * St::kUnit;
   // Guo forcing only where the lattice has a body force (uniform per
   // lattice); without one, u = j / rho and the source term is zero
   if (k.forced) {
@@ -285,7 +296,7 @@ __device__ __forceinline__ void std_collide(const StdParams& p,
   using C = typename St::C;
   C gv[Q];
 #pragma unroll
-  for (int i = 0; i < Q; ++i) gv[i] = St::cvt(ld.raw[i]);
+  for (int i = 0; i < Q; ++i) gv[i] = St::cvt_u(ld.raw[i]);
   // the node's Q outputs lie kTile elements apart: one 64-bit address, the
   // direction offsets become constant offsets of the stores
   typename St::T* const out0 = B + ld.own;
@@ -293,18 +304,15 @@ __device__ __forceinline__ void std_collide(const StdParams& p,
     // a body force moves 16-bit populations by less than their rounding
     // step: those lattices round stochastically (no effect at full
     // precision)
-    if constexpr (decltype(forced)::value) {
-      const uint32_t key = sr_key(St::kShifted ? sr_node_hash(ld.own) : 0u, i);
-      if (NT)
-        St::store_sr_nt(out0 + i * kTile, out, key);
-      else
-        St::store_sr(out0 + i * kTile, out, key);
-    } else {
-      if (NT)
-        St::store_nt(out0 + i * kTile, out);
-      else
-        St::store(out0 + i * kTile, out);
-    }
+    typename St::T v;
+    if constexpr (decltype(forced)::value)
+      v = St::round_sr_u(out, sr_key(St::kShifted ? sr_node_hash(ld.own) : 0u, i));
+    else
+      v = St::round_u(out);
+    if (NT)
+      gstore_nt(out0 + i * kTile, v);
+    else
+      gstore(out0 + i * kTile, v);
   });
 }
 
@@ -501,12 +509,12 @@ __device__ __forceinline__ void std_group_step(
   for (int k = 0; k < NV; ++k) {
     C gv[Q];
 #pragma unroll
-    for (int i = 0; i < Q; ++i) gv[i] = St::cvt(V[i][k]);
+    for (int i = 0; i < Q; ++i) gv[i] = St::cvt_u(V[i][k]);
     std_collide_node<Q, S, LID>(p, gv, m[k], solid[k], [&](auto forced, int i, C out) {
       if constexpr (decltype(forced)::value)
-        V[i][k] = St::round_sr(out, sr_key(St::kShifted ? sr_node_hash(own + k) : 0u, i));
+        V[i][k] = St::round_sr_u(out, sr_key(St::kShifted ? sr_node_hash(own + k) : 0u, i));
       else
-        V[i][k] = St::round(out);
+        V[i][k] = St::round_u(out);
     });
   }
   T* const out0 = B + own;

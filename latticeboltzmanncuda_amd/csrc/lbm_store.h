@@ -48,8 +48,21 @@ __device__ __forceinline__ uint32_t sr_key(uint32_t node_hash, int i) {
   return node_hash + (uint32_t)i * 0x85EBCA77u;
 }
 
+// Each storage type also exposes its values in stored units (cvt_u,
+// round_u, round_sr_u): the stored number itself, whose real value is
+// kUnit times it.  Only FP16 has kUnit != 1 (2^-12).  The collision works
+// in stored units, so the FP16 scale costs no multiply per population;
+// scaling by a power of two commutes with every rounding, so the results
+// are the same bits as in real units.
 template <int S>
 struct Store;
+
+// Types whose stored units are real units.
+#define LBM_UNIT_IS_VALUE                                                 \
+  static constexpr C kUnit = 1;                                            \
+  __device__ static C cvt_u(T v) { return cvt(v); }                        \
+  __device__ static T round_u(C v) { return round(v); }                    \
+  __device__ static T round_sr_u(C v, uint32_t key) { return round_sr(v, key); }
 
 template <>
 struct Store<LBM_STORE_F32> {
@@ -66,6 +79,7 @@ struct Store<LBM_STORE_F32> {
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, v); }
   __device__ static void store_sr(T* p, C v, uint32_t) { gstore(p, v); }
   __device__ static void store_sr_nt(T* p, C v, uint32_t) { gstore_nt(p, v); }
+  LBM_UNIT_IS_VALUE
 };
 
 // Standard-mode FP32 storage (internal code; the public storage type is
@@ -89,6 +103,7 @@ struct Store<kStoreF32Shifted> {
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, v); }
   __device__ static void store_sr(T* p, C v, uint32_t) { gstore(p, v); }
   __device__ static void store_sr_nt(T* p, C v, uint32_t) { gstore_nt(p, v); }
+  LBM_UNIT_IS_VALUE
 };
 
 // The storage code the kernels use for a lattice of this mode and public
@@ -112,6 +127,7 @@ struct Store<LBM_STORE_F64> {
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, v); }
   __device__ static void store_sr(T* p, C v, uint32_t) { gstore(p, v); }
   __device__ static void store_sr_nt(T* p, C v, uint32_t) { gstore_nt(p, v); }
+  LBM_UNIT_IS_VALUE
 };
 
 template <>
@@ -124,21 +140,29 @@ struct Store<LBM_STORE_F16> {
   __device__ static T rest(int, double) { return (T)0; }
   __device__ static C load(const T* p) { return cvt(gload(p)); }
   __device__ static T round(C v) { return (_Float16)(v * kScale); }
-  // the 13 bits FP32 -> FP16 drops get a random addend, then are cut (on
-  // the unscaled value: scaling by 2^12 keeps the mantissa, and the exact
-  // product is converted in the same instruction); a result in the FP16
-  // subnormal range (below 1.5e-8 unscaled) is rounded to nearest by the
-  // conversion
-  __device__ static T round_sr(C v, uint32_t key) {
-    const uint32_t b = __float_as_uint(v);
-    const uint32_t r = sr_bits(b, key) >> 19;  // 13 bits
-    // saturating add: a NaN's pattern cannot wrap round to a number
-    return (_Float16)(__uint_as_float(__builtin_elementwise_add_sat(b, r) & 0xFFFFE000u) * kScale);
-  }
+  __device__ static T round_sr(C v, uint32_t key) { return round_sr_u(v * kScale, key); }
   __device__ static void store(T* p, C v) { gstore(p, round(v)); }
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, round(v)); }
   __device__ static void store_sr(T* p, C v, uint32_t key) { gstore(p, round_sr(v, key)); }
   __device__ static void store_sr_nt(T* p, C v, uint32_t key) { gstore_nt(p, round_sr(v, key)); }
+
+  static constexpr C kUnit = kUnscale;
+  __device__ static C cvt_u(T v) { return (float)v; }
+  // where v comes straight from an fma (the collision's last step), the
+  // compiler folds the fma into this conversion (v_fma_mixlo_f16, under
+  // HIP's default contraction): the exact result is rounded once to FP16,
+  // not first to FP32
+  __device__ static T round_u(C v) { return (_Float16)v; }
+  // the 13 bits FP32 -> FP16 drops get a random addend, then are cut; a
+  // result in the FP16 subnormal range is rounded to nearest by the
+  // conversion.  The random bits come from the low 24 bits of the pattern,
+  // which scaling by 2^12 leaves alone (mantissa, and the exponent's parity)
+  __device__ static T round_sr_u(C v, uint32_t key) {
+    const uint32_t b = __float_as_uint(v);
+    const uint32_t r = sr_bits(b, key) >> 19;  // 13 bits
+    // saturating add: a NaN's pattern cannot wrap round to a number
+    return (_Float16)__uint_as_float(__builtin_elementwise_add_sat(b, r) & 0xFFFFE000u);
+  }
 };
 
 template <>
@@ -163,7 +187,10 @@ struct Store<LBM_STORE_BF16> {
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, round(v)); }
   __device__ static void store_sr(T* p, C v, uint32_t key) { gstore(p, round_sr(v, key)); }
   __device__ static void store_sr_nt(T* p, C v, uint32_t key) { gstore_nt(p, round_sr(v, key)); }
+  LBM_UNIT_IS_VALUE
 };
+
+#undef LBM_UNIT_IS_VALUE
 
 inline size_t storage_bytes(int storage) {
   switch (storage) {

@@ -154,8 +154,9 @@ def test_c3_fp16_taylor_green_vs_oracle_emulation():
     # the GPU's FP16 store path against the oracle's emulation of it: the
     # shifted deviation f_i - w_i times 2^12, rounded to nearest (no body
     # force), collided in FP32 from small terms.  The two FP32 evaluations
-    # differ in rounding order, so now and then a store rounds to the
-    # neighbouring FP16 value.
+    # differ in rounding order, and the GPU rounds the collision's last fma
+    # once to FP16 where the emulation rounds it to FP32 first, so now and
+    # then a store rounds to the neighbouring FP16 value.
     L = 64
     say(f"case: C3 TGV {L}^3 FP16 vs oracle FP16 emulation, 1 and 100 steps")
     rho, u = workloads.taylor_green_fields(L, L, 0.05)

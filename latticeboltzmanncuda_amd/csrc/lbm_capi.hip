@@ -271,6 +271,8 @@ int validate_config(const lbm_config* cfg) {
     if (!std::isfinite(cfg->force[a]) || !std::isfinite(cfg->lid_u[a]))
       return fail(LBM_ERR_INVALID, "body force and wall velocity must be finite");
   if (!std::isfinite(cfg->wall_rho)) return fail(LBM_ERR_INVALID, "wall density must be finite");
+  if (cfg->lattice == 9 && cfg->force[2] != 0.0f)
+    return fail(LBM_ERR_INVALID, "D2Q9 has no z axis: the body force's z component must be 0");
   if (cfg->lid_face < -1 || cfg->lid_face > 5)
     return fail(LBM_ERR_INVALID, "lid_face must be -1..5");
   if (cfg->lid_face >= 0) {
@@ -692,6 +694,8 @@ int lbm_set_body_force(lbm_ctx* c, const float force[3]) {
   if (!force) return fail(LBM_ERR_INVALID, "null force");
   for (int a = 0; a < 3; ++a)
     if (!std::isfinite(force[a])) return fail(LBM_ERR_INVALID, "body force must be finite");
+  if (c->cfg.lattice == 9 && force[2] != 0.0f)
+    return fail(LBM_ERR_INVALID, "D2Q9 has no z axis: the body force's z component must be 0");
   for (int a = 0; a < 3; ++a) c->cfg.force[a] = force[a];
   ++c->params_version;
   return LBM_OK;

@@ -49,6 +49,15 @@ __device__ __forceinline__ T bload16(__amdgpu_buffer_rsrc_t r, uint32_t off) {
   return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0));
 }
 
+// 16-bit store through a buffer resource at byte offset off; NT: with the
+// non-temporal hint (as gstore_nt)
+template <bool NT, class T>
+__device__ __forceinline__ void bstore16(__amdgpu_buffer_rsrc_t r, uint32_t off, T v) {
+  static_assert(sizeof(T) == 2, "16-bit stores only");
+  __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, (int)off, 0,
+                                        NT ? 2 : 0);
+}
+
 // a * b + c with one rounding
 __device__ __forceinline__ float fmadd(float a, float b, float c) {
   return __builtin_fmaf(a, b, c);

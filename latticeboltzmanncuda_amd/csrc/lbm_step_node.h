@@ -79,10 +79,22 @@ __device__ __forceinline__ float recip(float r) {
 template <int Q, int S>
 struct NodeLoad {
   typename Store<S>::T raw[Q];
-  uint32_t m;    // link mask (0 for interior and solid nodes)
-  uint32_t own;  // element index of (n, direction 0)
+  uint32_t m;          // link mask (0 for interior and solid nodes)
+  uint32_t own;        // element index of (n, direction 0)
+  uint32_t pop_bytes;  // bytes of one population array (16-bit storage)
   bool solid;
 };
+
+// b where bit i of m is set, else a: bit i spread over a word by a signed
+// bit-field extract, then a bit-field insert -- two instructions where the
+// compiler's compare-and-select takes three (the insert is written out:
+// the compiler turns the equivalent and/or expression back into the select)
+__device__ __forceinline__ uint32_t select_bit(uint32_t m, int i, uint32_t b, uint32_t a) {
+  const uint32_t sel = (uint32_t)__builtin_amdgcn_sbfe((int)m, i, 1);
+  uint32_t r;
+  __asm__("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(sel), "v"(b), "v"(a));
+  return r;
+}
 
 // Phase 1: issue every load of node n (no value is used here, so the loads
 // of several nodes can be in flight together).
@@ -113,31 +125,48 @@ __device__ __forceinline__ void std_gather(const Grid& g,
   const uint32_t oz[3] = {z + 1 == g.nz ? 1u - g.nz : 1u, 0u,
                           z == 0 ? g.nz - 1u : 0u - 1u};
 
+  using T = typename Store<S>::T;
   ld.solid = c == kClassSolid;
   ld.m = c == kClassBoundary ? gload(mask + n) : 0u;
   ld.own = pidx<Q>(n, 0);
-  // solid lanes load nothing: their values are left unspecified (no
-  // instruction spent on them), the collision replaces their outputs with
-  // the rest state
+  ld.pop_bytes = sizeof(T) == 2 ? (g.n + kTile - 1u) / kTile * (uint32_t)(Q * kTile * 2) : 0u;
+  // solid lanes' values are unspecified (the collision replaces their
+  // outputs with the rest state).  With FP32 / FP64 storage solid lanes
+  // load nothing.  With 16-bit storage (bound by instruction issue) every
+  // lane loads: the lines solid lanes read are mostly those their fluid
+  // neighbours read, and skipping them would cost instructions to give
+  // the skipped registers a value on every wavefront
 #pragma unroll
   for (int i = 0; i < Q; ++i) ld.raw[i] = __builtin_nondeterministic_value(ld.raw[i]);
-  if (!ld.solid) {
-    using T = typename Store<S>::T;
+  if (sizeof(T) == 2 || !ld.solid) {
     if constexpr (sizeof(T) == 2) {
       // 16-bit storage, in byte offsets (below 2^32: lbm_create keeps
       // element indices below 2^31) through a buffer resource over the
       // population array: element (m, i) lies at
-      //   2 pidx(m, i) = 2 m + (m / 64) (Q - 1) 128 + 128 i,
-      // one 24-bit multiply-add per source node.  The part of the offset
+      //   2 pidx(m, i) = 2 m + (2 m / 128) (Q - 1) 128 + 128 i,
+      // formed from the doubled index 2 m (the offsets to the neighbours
+      // doubled once per node) with a shift and one 24-bit multiply-add
+      // per source node.  The part of the offset
       // that the pulled address (direction i of the source node) and the
       // bounced one (direction INV[i] of the node itself) share,
       // 128 min(i, INV[i]), is a constant of the load instruction.
       constexpr uint32_t kRow = (uint32_t)kTile * 2u;  // bytes of one direction's run
-      const auto rsrc = buffer_rsrc(A, (g.n + kTile - 1u) / kTile * (uint32_t)Q * kRow);
+      const auto rsrc = buffer_rsrc(A, ld.pop_bytes);
       const uint32_t own_b = 2u * ld.own;
+      // the doubled offsets, selected from doubled (scalar) values
+      const uint32_t p2 = 2u * plane, n2 = 2u * g.n, z2 = 2u * g.nz;
+      const uint32_t ox2[3] = {x + 1 == g.nx ? p2 - n2 : p2, 0u, x == 0 ? n2 - p2 : 0u - p2};
+      const uint32_t oy2[3] = {y + 1 == g.ny ? z2 - p2 : z2, 0u, y == 0 ? p2 - z2 : 0u - z2};
+      const uint32_t oz2[3] = {2u * oz[0], 0u, 2u * oz[2]};
+      // (Q - 1) kRow passed through an empty asm: for D2Q9 it is a power of
+      // two, which the compiler would turn into a shift and a mask -- one
+      // instruction more than the multiply-add
+      uint32_t run_bytes = (uint32_t)(Q - 1) * kRow;
+      asm("" : "+s"(run_bytes));
       auto pull_b = [&](int i) {
-        const uint32_t src = n + ox[L::E[i][0] + 1] + oy[L::E[i][1] + 1] + oz[L::E[i][2] + 1];
-        return __umul24(src / kTile, (uint32_t)(Q - 1) * kRow) + 2u * src;
+        const uint32_t src2 =
+            2u * n + ox2[L::E[i][0] + 1] + oy2[L::E[i][1] + 1] + oz2[L::E[i][2] + 1];
+        return __umul24(src2 / kRow, run_bytes) + src2;
       };
       // a wavefront without a bounced link among its nodes (a third of
       // them in the batching workload, more in large lattices) skips the
@@ -152,8 +181,7 @@ __device__ __forceinline__ void std_gather(const Grid& g,
           const uint32_t a = pull_b(i) + ((uint32_t)i * kRow - lo);
           const uint32_t b = own_b + ((uint32_t)L::INV[i] * kRow - lo);
           // bounced link: read the node's own opposite population instead
-          const uint32_t off = ((ld.m >> i) & 1u) ? b : a;
-          ld.raw[i] = bload16<T>(rsrc, off + lo);
+          ld.raw[i] = bload16<T>(rsrc, select_bit(ld.m, i, b, a) + lo);
         }
       }
     } else {
@@ -179,7 +207,8 @@ struct Forced {
 
 // BGK relaxation towards the equilibrium of (rho, u), plus the Guo source
 // term when FORCED; emit(Forced<FORCED>{}, i, out) receives the Q outputs
-// in the stored form and stored units (solid nodes: the rest state).  gv
+// in the stored form and stored units (solid nodes: the rest state;
+// SOLID false: the caller's wavefront has no solid node).  gv
 // holds the inputs in stored units, k the constants in stored units; rs is
 // rho (full storage) or drho = rho - 1 (shifted storage), in real units.
 // Shifted storage forms feq - f = w rho t - w - g as
@@ -187,7 +216,7 @@ struct Forced {
 // from small terms only: no intermediate is near 1 or near w, so the
 // rounding error of the collision scales with the deviations and a body
 // force's small per-step increments are not lost.
-template <int Q, int S, bool FORCED, class Emit>
+template <int Q, int S, bool FORCED, bool SOLID, class Emit>
 __device__ __forceinline__ void std_relax(const StepCoef<typename Store<S>::C>& k,
                                           const typename Store<S>::C (&gv)[Q],
                                           typename Store<S>::C rs, typename Store<S>::C ux,
@@ -198,9 +227,15 @@ __device__ __forceinline__ void std_relax(const StepCoef<typename Store<S>::C>& 
   using St = Store<S>;
   using C = typename St::C;
   constexpr bool kSh = St::kShifted;
-  const C usq = ux * ux + uy * uy + uz * uz;
+  const C usq = L::D == 2 ? ux * ux + uy * uy : ux * ux + uy * uy + uz * uz;
   // t - 1 (shifted) or t = 1 - 1.5 u.u + 3 e.u + 4.5 (e.u)^2 (full)
   const C base = kSh ? (C)-1.5 * usq : (C)1 - (C)1.5 * usq;
+  // 16-bit storage forms (t-1) rho + drho with rho = 1 + drho rounded once
+  // per node: one fma per direction, an error of 6e-8 relative to the
+  // term, far below the 16-bit stored value's resolution (FP32 storage
+  // keeps the exact small-term form below)
+  constexpr bool k16 = sizeof(typename St::T) == 2;
+  const C rho = (C)1 + rs;
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     // e.u from the nonzero components only (-0 + v == v exactly)
@@ -213,16 +248,20 @@ __device__ __forceinline__ void std_relax(const StepCoef<typename Store<S>::C>& 
     const C t = fmadd(fmadd((C)4.5, eu, (C)3), eu, base);
     // feq - f in the stored form: w ((t-1) + drho + drho (t-1)) - g
     // (shifted, t here is t - 1), w rho t - f (full)
-    const C d = kSh ? fmadd(wu, fmadd(rs, t, t + rs), -gv[i]) : (w * rs) * t - gv[i];
+    const C d = k16 ? fmadd(wu, fmadd(t, rho, rs), -gv[i])
+                : kSh ? fmadd(wu, fmadd(rs, t, t + rs), -gv[i])
+                      : (w * rs) * t - gv[i];
     C out = fmadd(k.omega, d, gv[i]);
     if (FORCED) {
       // Guo source (1 - omega/2) w (3 (e.F - u.F) + 9 (e.u)(e.F)), added
-      // term by term (each step takes one per-lattice constant)
-      out = out + k.gf3[i];
+      // term by term (each step takes one per-lattice constant); the rest
+      // direction has e = 0, so only its -3 u.F term is nonzero
+      const bool kRest = L::E[i][0] == 0 && L::E[i][1] == 0 && L::E[i][2] == 0;
+      if (!kRest) out = out + k.gf3[i];
       out = fmadd(k.gw[i], m3uF, out);
-      out = fmadd(k.gf9[i], eu, out);
+      if (!kRest) out = fmadd(k.gf9[i], eu, out);
     }
-    out = solid ? (kSh ? (C)0 : w) : out;
+    if (SOLID) out = solid ? (kSh ? (C)0 : w) : out;
     emit(Forced<FORCED>{}, i, out);
   }
 }
@@ -248,12 +287,17 @@ __device__ __forceinline__ void std_collide_node(const StdParams& p,
   // only nodes next to the moving face have such links, so a wavefront
   // without one skips the selection (the branch is uniform)
   if (LID) {
-    const bool lidnode = (m & LBM_MASK_LID) != 0u;
-    if (__any(lidnode)) {
+    const uint32_t ml = (m & LBM_MASK_LID) ? m : 0u;  // links of a node next to it
+    if (__builtin_amdgcn_ballot_w64(ml != 0u) != 0) {
 #pragma unroll
       for (int i = 1; i < Q; ++i) {  // the rest direction never crosses a face
-        const bool wall = lidnode && ((m >> i) & 1u);
-        gv[i] += wall ? k.lid[i] : (C)0;
+        if constexpr (sizeof(C) == 4) {
+          // the term or +0: its bits masked by bit i spread over a word
+          const uint32_t sel = (uint32_t)__builtin_amdgcn_sbfe((int)ml, i, 1);
+          gv[i] += __uint_as_float(sel & __float_as_uint(k.lid[i]));
+        } else {
+          gv[i] += ((ml >> i) & 1u) ? k.lid[i] : (C)0;
+        }
       }
     }
   }
@@ -276,14 +320,27 @@ __device__ __forceinline__ void std_collide_node(const StdParams& p,
   const C ir = recip(kSh ? (C)1 + rs : rs) * St::kUnit;
   // Guo forcing only where the lattice has a body force (uniform per
   // lattice); without one, u = j / rho and the source term is zero
+  // a wavefront without a solid node skips the rest-state selection (the
+  // branch is uniform)
+  const bool any_solid = __builtin_amdgcn_ballot_w64(solid) != 0;
+  // D2Q9 has no z velocity (its z force is 0: lbm_create checks)
+  constexpr bool k2D = L::D == 2;
   if (k.forced) {
     const C ux = (jx + k.half_force[0]) * ir;
     const C uy = (jy + k.half_force[1]) * ir;
-    const C uz = (jz + k.half_force[2]) * ir;
-    const C m3uF = (C)-3 * (ux * k.force[0] + uy * k.force[1] + uz * k.force[2]);
-    std_relax<Q, S, true>(k, gv, rs, ux, uy, uz, m3uF, solid, emit);
+    const C uz = k2D ? (C)0 : (jz + k.half_force[2]) * ir;
+    const C m3uF = k2D ? (C)-3 * (ux * k.force[0] + uy * k.force[1])
+                       : (C)-3 * (ux * k.force[0] + uy * k.force[1] + uz * k.force[2]);
+    if (any_solid)
+      std_relax<Q, S, true, true>(k, gv, rs, ux, uy, uz, m3uF, solid, emit);
+    else
+      std_relax<Q, S, true, false>(k, gv, rs, ux, uy, uz, m3uF, solid, emit);
   } else {
-    std_relax<Q, S, false>(k, gv, rs, jx * ir, jy * ir, jz * ir, (C)0, solid, emit);
+    const C ux = jx * ir, uy = jy * ir, uz = k2D ? (C)0 : jz * ir;
+    if (any_solid)
+      std_relax<Q, S, false, true>(k, gv, rs, ux, uy, uz, (C)0, solid, emit);
+    else
+      std_relax<Q, S, false, false>(k, gv, rs, ux, uy, uz, (C)0, solid, emit);
   }
 }
 
@@ -297,19 +354,24 @@ __device__ __forceinline__ void std_collide(const StdParams& p,
   C gv[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) gv[i] = St::cvt_u(ld.raw[i]);
-  // the node's Q outputs lie kTile elements apart: one 64-bit address, the
-  // direction offsets become constant offsets of the stores
-  typename St::T* const out0 = B + ld.own;
+  // the node's Q outputs lie kTile elements apart: one address (16-bit
+  // storage: a byte offset into a buffer resource), the direction offsets
+  // become constant offsets of the stores
+  using T = typename St::T;
+  T* const out0 = B + ld.own;
+  const auto rsrc = buffer_rsrc(B, ld.pop_bytes);
   std_collide_node<Q, S, LID>(p, gv, ld.m, ld.solid, [&](auto forced, int i, C out) {
     // a body force moves 16-bit populations by less than their rounding
     // step: those lattices round stochastically (no effect at full
     // precision)
-    typename St::T v;
+    T v;
     if constexpr (decltype(forced)::value)
       v = St::round_sr_u(out, sr_key(St::kShifted ? sr_node_hash(ld.own) : 0u, i));
     else
       v = St::round_u(out);
-    if (NT)
+    if constexpr (sizeof(T) == 2)
+      bstore16<NT>(rsrc, 2u * ld.own + (uint32_t)(i * kTile * 2), v);
+    else if (NT)
       gstore_nt(out0 + i * kTile, v);
     else
       gstore(out0 + i * kTile, v);

@@ -33,20 +33,23 @@
 
 namespace lbm {
 
-// Random bits for stochastic rounding of the float with bit pattern b:
-// key = node hash + direction constant (sr_key).  One 24-bit multiply-add,
-// the low 24 bits of the value times an odd constant plus the key: for a
-// given value the key, uniform over nodes, makes the bits uniform, and a
-// node's bits change from step to step with its values.  Callers take the
-// top bits (taking low ones would let the compiler drop the 24-bit operand
-// mask and fall back to the quarter-rate 32-bit multiply).
-__device__ __forceinline__ uint32_t sr_bits(uint32_t b, uint32_t key) {
-  return __umul24(b, 0x2C1B3Du) + key;
-}
-__device__ __forceinline__ uint32_t sr_node_hash(uint32_t own) { return own * 0x9E3779B1u; }
-__device__ __forceinline__ uint32_t sr_key(uint32_t node_hash, int i) {
-  return node_hash + (uint32_t)i * 0x85EBCA77u;
-}
+// Random bits for stochastic rounding of the float with bit pattern b, the
+// population of direction i of a node: one 24-bit multiply-add, the low 24
+// bits of the value times the direction's odd multiplier (sr_mul) plus the
+// node hash (sr_node_hash, the low 24 bits of the node's element index
+// times an odd constant).  For a given value the hash, uniform over nodes,
+// makes the bits uniform; the multipliers decorrelate a node's directions;
+// a node's bits change from step to step with its values.  Callers take
+// the top bits (taking low ones would let the compiler drop the 24-bit
+// operand mask and fall back to the quarter-rate 32-bit multiply).
+struct SrKey {
+  uint32_t mul;   // odd, below 2^24
+  uint32_t hash;  // node hash
+};
+constexpr uint32_t sr_mul(int i) { return ((0x9E3779B9u * (uint32_t)(i + 1)) >> 8) | 1u; }
+__device__ __forceinline__ uint32_t sr_node_hash(uint32_t own) { return __umul24(own, 0x9E3779u); }
+__device__ __forceinline__ SrKey sr_key(uint32_t node_hash, int i) { return {sr_mul(i), node_hash}; }
+__device__ __forceinline__ uint32_t sr_bits(uint32_t b, SrKey k) { return __umul24(b, k.mul) + k.hash; }
 
 // Each storage type also exposes its values in stored units (cvt_u,
 // round_u, round_sr_u): the stored number itself, whose real value is
@@ -62,7 +65,7 @@ struct Store;
   static constexpr C kUnit = 1;                                            \
   __device__ static C cvt_u(T v) { return cvt(v); }                        \
   __device__ static T round_u(C v) { return round(v); }                    \
-  __device__ static T round_sr_u(C v, uint32_t key) { return round_sr(v, key); }
+  __device__ static T round_sr_u(C v, SrKey key) { return round_sr(v, key); }
 
 template <>
 struct Store<LBM_STORE_F32> {
@@ -74,11 +77,11 @@ struct Store<LBM_STORE_F32> {
   __device__ static C load(const T* p) { return gload(p); }
   __device__ static T round(C v) { return v; }
   // full-precision storage rounds to nearest
-  __device__ static T round_sr(C v, uint32_t) { return v; }
+  __device__ static T round_sr(C v, SrKey) { return v; }
   __device__ static void store(T* p, C v) { gstore(p, v); }
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, v); }
-  __device__ static void store_sr(T* p, C v, uint32_t) { gstore(p, v); }
-  __device__ static void store_sr_nt(T* p, C v, uint32_t) { gstore_nt(p, v); }
+  __device__ static void store_sr(T* p, C v, SrKey) { gstore(p, v); }
+  __device__ static void store_sr_nt(T* p, C v, SrKey) { gstore_nt(p, v); }
   LBM_UNIT_IS_VALUE
 };
 
@@ -98,11 +101,11 @@ struct Store<kStoreF32Shifted> {
   __device__ static T rest(int, double) { return 0.0f; }
   __device__ static C load(const T* p) { return gload(p); }
   __device__ static T round(C v) { return v; }
-  __device__ static T round_sr(C v, uint32_t) { return v; }
+  __device__ static T round_sr(C v, SrKey) { return v; }
   __device__ static void store(T* p, C v) { gstore(p, v); }
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, v); }
-  __device__ static void store_sr(T* p, C v, uint32_t) { gstore(p, v); }
-  __device__ static void store_sr_nt(T* p, C v, uint32_t) { gstore_nt(p, v); }
+  __device__ static void store_sr(T* p, C v, SrKey) { gstore(p, v); }
+  __device__ static void store_sr_nt(T* p, C v, SrKey) { gstore_nt(p, v); }
   LBM_UNIT_IS_VALUE
 };
 
@@ -122,11 +125,11 @@ struct Store<LBM_STORE_F64> {
   __device__ static C load(const T* p) { return gload(p); }
   __device__ static T round(C v) { return v; }
   // full-precision storage rounds to nearest
-  __device__ static T round_sr(C v, uint32_t) { return v; }
+  __device__ static T round_sr(C v, SrKey) { return v; }
   __device__ static void store(T* p, C v) { gstore(p, v); }
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, v); }
-  __device__ static void store_sr(T* p, C v, uint32_t) { gstore(p, v); }
-  __device__ static void store_sr_nt(T* p, C v, uint32_t) { gstore_nt(p, v); }
+  __device__ static void store_sr(T* p, C v, SrKey) { gstore(p, v); }
+  __device__ static void store_sr_nt(T* p, C v, SrKey) { gstore_nt(p, v); }
   LBM_UNIT_IS_VALUE
 };
 
@@ -140,11 +143,11 @@ struct Store<LBM_STORE_F16> {
   __device__ static T rest(int, double) { return (T)0; }
   __device__ static C load(const T* p) { return cvt(gload(p)); }
   __device__ static T round(C v) { return (_Float16)(v * kScale); }
-  __device__ static T round_sr(C v, uint32_t key) { return round_sr_u(v * kScale, key); }
+  __device__ static T round_sr(C v, SrKey key) { return round_sr_u(v * kScale, key); }
   __device__ static void store(T* p, C v) { gstore(p, round(v)); }
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, round(v)); }
-  __device__ static void store_sr(T* p, C v, uint32_t key) { gstore(p, round_sr(v, key)); }
-  __device__ static void store_sr_nt(T* p, C v, uint32_t key) { gstore_nt(p, round_sr(v, key)); }
+  __device__ static void store_sr(T* p, C v, SrKey key) { gstore(p, round_sr(v, key)); }
+  __device__ static void store_sr_nt(T* p, C v, SrKey key) { gstore_nt(p, round_sr(v, key)); }
 
   static constexpr C kUnit = kUnscale;
   __device__ static C cvt_u(T v) { return (float)v; }
@@ -157,7 +160,7 @@ struct Store<LBM_STORE_F16> {
   // result in the FP16 subnormal range is rounded to nearest by the
   // conversion.  The random bits come from the low 24 bits of the pattern,
   // which scaling by 2^12 leaves alone (mantissa, and the exponent's parity)
-  __device__ static T round_sr_u(C v, uint32_t key) {
+  __device__ static T round_sr_u(C v, SrKey key) {
     const uint32_t b = __float_as_uint(v);
     const uint32_t r = sr_bits(b, key) >> 19;  // 13 bits
     // saturating add: a NaN's pattern cannot wrap round to a number
@@ -179,14 +182,14 @@ struct Store<LBM_STORE_BF16> {
     // 0xFFFFFFFF would otherwise wrap round to a signed zero
     return (uint16_t)(__builtin_elementwise_add_sat(u, 0x7fffu + ((u >> 16) & 1u)) >> 16);
   }
-  __device__ static T round_sr(C v, uint32_t key) {
+  __device__ static T round_sr(C v, SrKey key) {
     const uint32_t u = __float_as_uint(v);
     return (uint16_t)(__builtin_elementwise_add_sat(u, sr_bits(u, key) >> 16) >> 16);
   }
   __device__ static void store(T* p, C v) { gstore(p, round(v)); }
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, round(v)); }
-  __device__ static void store_sr(T* p, C v, uint32_t key) { gstore(p, round_sr(v, key)); }
-  __device__ static void store_sr_nt(T* p, C v, uint32_t key) { gstore_nt(p, round_sr(v, key)); }
+  __device__ static void store_sr(T* p, C v, SrKey key) { gstore(p, round_sr(v, key)); }
+  __device__ static void store_sr_nt(T* p, C v, SrKey key) { gstore_nt(p, round_sr(v, key)); }
   LBM_UNIT_IS_VALUE
 };
 

@@ -848,15 +848,18 @@ float orc_round_bf16(float x) {
  * stores it: FP16 holds v * 2^12 (so deviations down to 1.5e-8 are normal
  * binary16 numbers), BF16 holds v.  Lattices with a body force round
  * stochastically: the bits the conversion drops get a random addend before
- * they are cut, the addend a hash of the value's float pattern, of the
- * node's element index in the GPU's blocked layout ((n / 64) * Q * 64 +
- * n % 64) and of the direction -- the GPU's rule, so the emulation rounds
- * with the same distribution (its floats can differ from the GPU's by the
- * order of operations, so individual roundings can differ).  Returns the
- * stored value, unscaled. */
-static uint32_t orc_sr_key(int64_t n, int q, int i) {
+ * they are cut -- the GPU's rule, so the emulation rounds with the same
+ * distribution (its floats can differ from the GPU's by the order of
+ * operations, so individual roundings can differ).  The addend is the top
+ * bits of (low 24 bits of the value's float pattern) x (an odd 24-bit
+ * multiplier of the direction) + (a node hash: the low 24 bits of the
+ * node's element index in the GPU's blocked layout, (n / 64) * Q * 64 +
+ * n % 64, times an odd constant).  Returns the stored value, unscaled. */
+uint32_t orc_sr_mul(int i) { return ((0x9E3779B9u * (uint32_t)(i + 1)) >> 8) | 1u; }
+
+static uint32_t orc_sr_hash(int64_t n, int q) {
   const uint32_t own = (uint32_t)((n >> 6) * (int64_t)q * 64 + (n & 63));
-  return own * 0x9E3779B1u + (uint32_t)i * 0x85EBCA77u;
+  return (own & 0xffffffu) * 0x9E3779u;
 }
 
 static uint32_t orc_sat_add(uint32_t a, uint32_t b) {
@@ -864,15 +867,16 @@ static uint32_t orc_sat_add(uint32_t a, uint32_t b) {
   return s < a ? 0xffffffffu : s;
 }
 
-static float orc_store16(int storage, float v, int sr, uint32_t key) {
+static float orc_store16(int storage, float v, int sr, uint32_t mul, uint32_t hash) {
   if (storage == 2) {
     if (sr) {
       /* the 13 bits the FP16 conversion drops, cut on the unscaled value
-       * (scaling by 2^12 keeps the mantissa): the top 13 bits of the low
-       * 32 bits of (24-bit value bits) x constant + key */
+       * (scaling by 2^12 keeps the mantissa and the exponent's parity, so
+       * the low 24 bits the GPU hashes are the same): the top 13 bits of
+       * the low 32 bits of (24-bit value bits) x multiplier + hash */
       uint32_t b;
       memcpy(&b, &v, 4);
-      const uint32_t r = ((b & 0xffffffu) * 0x2C1B3Du + key) >> 19;
+      const uint32_t r = ((b & 0xffffffu) * mul + hash) >> 19;
       b = orc_sat_add(b, r) & 0xffffe000u;
       memcpy(&v, &b, 4);
     }
@@ -882,7 +886,7 @@ static float orc_store16(int storage, float v, int sr, uint32_t key) {
     if (!sr) return orc_round_bf16(v);
     uint32_t b;
     memcpy(&b, &v, 4);
-    const uint32_t r = ((b & 0xffffffu) * 0x2C1B3Du + key) >> 16;
+    const uint32_t r = ((b & 0xffffffu) * mul + hash) >> 16;
     b = orc_sat_add(b, r) & 0xffff0000u;
     float o;
     memcpy(&o, &b, 4);
@@ -891,8 +895,8 @@ static float orc_store16(int storage, float v, int sr, uint32_t key) {
   return v;
 }
 
-float orc_store16_sr(int storage, float v, uint32_t key) {
-  return orc_store16(storage, v, 1, key);
+float orc_store16_sr(int storage, float v, int direction, uint32_t hash) {
+  return orc_store16(storage, v, 1, orc_sr_mul(direction), hash);
 }
 
 /* One pull step of the standard model for one node, float compute.
@@ -992,7 +996,8 @@ static void std_step_f32(int q, int nx, int ny, int nz, const float* A,
  *   drho = sum g, u = (sum e g + F/2) / (1 + drho),
  *   tm1 = (4.5 e.u + 3) e.u - 1.5 u.u                          (t - 1),
  *   g*_i = g_i + omega (w_i (drho tm1 + (tm1 + drho)) - g_i)
- *                         (= w rho t - w - g for rho = 1 + drho)
+ *                         (= w rho t - w - g for rho = 1 + drho;
+ *                          16-bit storage: w_i (tm1 rho + drho))
  *          [+ gf3_i + gw_i (-3 u.F) + gf9_i e.u   with a body force]
  * (gw = (1 - omega/2) w, gf3 = 3 gw e.F, gf9 = 9 gw e.F), each operation
  * rounded to float; solid nodes store the rest state g = 0.  Working on
@@ -1057,7 +1062,8 @@ static void std_step_shifted(int q, int nx, int ny, int nz, const float* A,
           jy += (float)E[i][1] * g[i];
           jz += (float)E[i][2] * g[i];
         }
-        const float ir = 1.0f / (1.0f + drho);
+        const float rho = 1.0f + drho;
+        const float ir = 1.0f / rho;
         const float ux = (jx + 0.5f * F[0]) * ir, uy = (jy + 0.5f * F[1]) * ir,
                     uz = (jz + 0.5f * F[2]) * ir;
         const float m15usq = -1.5f * (ux * ux + uy * uy + uz * uz);
@@ -1065,16 +1071,20 @@ static void std_step_shifted(int q, int nx, int ny, int nz, const float* A,
         for (int i = 0; i < q; ++i) {
           const float eu = (float)E[i][0] * ux + (float)E[i][1] * uy + (float)E[i][2] * uz;
           /* t - 1, and w rho t - w = w ((t - 1) + drho + drho (t - 1)), all
-           * formed from small terms */
+           * formed from small terms -- for 16-bit storage as
+           * w ((t - 1) rho + drho) with rho = 1 + drho rounded once, whose
+           * error (6e-8 of the term) lies far below the 16-bit value's
+           * resolution */
           const float tm1 = (4.5f * eu + 3.0f) * eu + m15usq;
-          const float d = w[i] * (drho * tm1 + (tm1 + drho)) - g[i];
+          const float d = (storage == 2 || storage == 3) ? w[i] * (tm1 * rho + drho) - g[i]
+                                  : w[i] * (drho * tm1 + (tm1 + drho)) - g[i];
           float out = omega * d + g[i];
           if (forced) {
             out = out + gf3[i];
             out = gw[i] * m3uF + out;
             out = gf9[i] * eu + out;
           }
-          B[i * N + n] = orc_store16(storage, out, forced, forced ? orc_sr_key(n, q, i) : 0u);
+          B[i * N + n] = orc_store16(storage, out, forced, orc_sr_mul(i), orc_sr_hash(n, q));
         }
       }
 }
@@ -1107,7 +1117,7 @@ void orc_std_run_f32(int q, int nx, int ny, int nz, float* fA,
   for (int i = 0; i < q; ++i)
     for (int64_t n = 0; n < N; ++n) {
       const float v = fA[i * N + n];
-      A[i * N + n] = storage ? orc_store16(storage, v - (float)W[i], 0, 0u) : v;
+      A[i * N + n] = storage ? orc_store16(storage, v - (float)W[i], 0, 0u, 0u) : v;
     }
   memcpy(B, A, bytes);
   for (int s = 0; s < steps; ++s) {

@@ -184,10 +184,12 @@ void orc_std_moments_f64(int q, int nx, int ny, int nz, const double* f,
 /* float <-> storage rounding used by the shifted storage emulation */
 float orc_round_fp16(float x);
 float orc_round_bf16(float x);
-/* the stored (unscaled) 16-bit value of the shifted deviation v with
- * stochastic rounding under the node/direction key `key`, storage 2 =
- * FP16 (times 2^12), 3 = BF16 -- as forced lattices store it */
-float orc_store16_sr(int storage, float v, uint32_t key);
+/* the stored (unscaled) 16-bit value of the shifted deviation v of
+ * direction `direction` with stochastic rounding under the node hash
+ * `hash`, storage 2 = FP16 (times 2^12), 3 = BF16 -- as forced lattices
+ * store it; orc_sr_mul is the direction's multiplier in the rounding hash */
+float orc_store16_sr(int storage, float v, int direction, uint32_t hash);
+uint32_t orc_sr_mul(int direction);
 
 /* ---------------- random spheres ---------------- */
 uint64_t orc_mt64_nth(uint64_t seed, int64_t n); /* n-th output, 1-based */

@@ -114,7 +114,7 @@ def _declare(L: C.CDLL) -> None:
     L.orc_round_fp16.restype = C.c_float
     L.orc_round_bf16.argtypes = [C.c_float]
     L.orc_round_bf16.restype = C.c_float
-    L.orc_store16_sr.argtypes = [C.c_int, C.c_float, C.c_uint32]
+    L.orc_store16_sr.argtypes = [C.c_int, C.c_float, C.c_int, C.c_uint32]
     L.orc_store16_sr.restype = C.c_float
     L.orc_mt64_nth.argtypes = [C.c_uint64, C.c_int64]
     L.orc_mt64_nth.restype = C.c_uint64
@@ -349,10 +349,11 @@ def round_bf16(x: float) -> float:
     return lib().orc_round_bf16(x)
 
 
-def store16_sr(storage: int, v: float, key: int) -> float:
-    """The stored (unscaled) value of v with stochastic rounding under the
-    node/direction key (storage STORE_F16_SHIFTED or STORE_BF16_SHIFTED)."""
-    return lib().orc_store16_sr(storage, v, key & 0xFFFFFFFF)
+def store16_sr(storage: int, v: float, hash_: int, direction: int = 0) -> float:
+    """The stored (unscaled) value of v, a population of `direction`, with
+    stochastic rounding under the node hash (storage STORE_F16_SHIFTED or
+    STORE_BF16_SHIFTED)."""
+    return lib().orc_store16_sr(storage, v, direction, hash_ & 0xFFFFFFFF)
 
 
 def mt64_nth(seed: int, n: int) -> int:

@@ -48,6 +48,14 @@ def test_error_reporting_without_gpu_work():
     assert lib.lbm_create(C.byref(cfg), None, C.byref(h)) == 1
     cfg = capi.make_config(4, 4, 4, lid_face=3, periodic=(0, 1, 0))
     assert lib.lbm_create(C.byref(cfg), None, C.byref(h)) == 1
+    # a 2-D lattice has no z axis, so no z force (its x / y force is fine)
+    cfg = capi.make_config(4, 4, 1, lattice=9, force=(0, 0, 1e-6))
+    assert lib.lbm_create(C.byref(cfg), None, C.byref(h)) == 1
+    assert b"z component" in lib.lbm_last_error()
+    with pytest.raises(capi.LbmError):
+        capi.cell_reynolds(cfg)
+    assert capi.cell_reynolds(capi.make_config(
+        4, 4, 1, lattice=9, force=(1e-6, 0, 0), periodic=(1, 0, 0))) > 0
     assert lib.lbm_step(None, 1) == 1
 
 

@@ -634,14 +634,18 @@ def _bf16_neighbours(v):
 @pytest.mark.parametrize("storage", [O.STORE_F16_SHIFTED, O.STORE_BF16_SHIFTED])
 def test_stochastic_rounding_is_unbiased(storage):
     # The 16-bit store of a forced lattice: every result is one of the two
-    # neighbouring 16-bit values, and over node keys (uniform, as the node
-    # hash makes them) its mean is the stored value itself -- rounding
-    # to nearest would be biased by up to half a unit for the same values.
-    # A wrong shift, mask or dropped key fails one of the two.
+    # neighbouring 16-bit values, and over the nodes of a lattice (the node
+    # hash: low 24 bits of the element index in the blocked layout times
+    # an odd constant) its mean is the stored value itself -- rounding to
+    # nearest would be biased by up to half a unit for the same values.
+    # A wrong shift, mask, multiplier or dropped hash fails one of the two.
     rng = np.random.default_rng(11)
-    keys = rng.integers(0, 2 ** 32, 3000, dtype=np.uint64)
+    n = np.arange(3000, dtype=np.uint64) * 7 + 5  # nodes of a D3Q19 lattice
+    own = (n >> np.uint64(6)) * np.uint64(19 * 64) + (n & np.uint64(63))
+    hashes = ((own & np.uint64(0xFFFFFF)) * np.uint64(0x9E3779)) & np.uint64(0xFFFFFFFF)
     scale = 4096.0 if storage == O.STORE_F16_SHIFTED else 1.0
-    for v in (rng.choice([-1, 1], 24) * 10 ** rng.uniform(-6, -1, 24)).astype(np.float32):
+    vals = (rng.choice([-1, 1], 24) * 10 ** rng.uniform(-6, -1, 24)).astype(np.float32)
+    for j, v in enumerate(vals):
         v = float(v)
         if storage == O.STORE_F16_SHIFTED:
             lo, hi = _f16_neighbours(np.float32(v * scale))
@@ -649,8 +653,9 @@ def test_stochastic_rounding_is_unbiased(storage):
             lo, hi = _bf16_neighbours(v)
         if lo == hi:
             continue
-        got = np.array([O.store16_sr(storage, v, int(k)) * scale for k in keys])
+        d = j % 19  # each direction's multiplier
+        got = np.array([O.store16_sr(storage, v, int(h), d) * scale for h in hashes])
         assert np.all((got == lo) | (got == hi)), (v, set(got) - {lo, hi})
         p = (v * scale - lo) / (hi - lo)  # exact rounding-up probability
-        sd = (hi - lo) * np.sqrt(p * (1 - p) / len(keys))
+        sd = (hi - lo) * np.sqrt(p * (1 - p) / len(hashes))
         assert abs(got.mean() - v * scale) <= 4.5 * sd + 1e-12 * abs(v * scale), (v, got.mean(), sd)

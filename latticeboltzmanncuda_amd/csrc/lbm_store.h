@@ -47,7 +47,9 @@ struct SrKey {
   uint32_t hash;  // node hash
 };
 constexpr uint32_t sr_mul(int i) { return ((0x9E3779B9u * (uint32_t)(i + 1)) >> 8) | 1u; }
-__device__ __forceinline__ uint32_t sr_node_hash(uint32_t own) { return __umul24(own, 0x9E3779u); }
+// (the node hash's constant differs from every direction's multiplier, or
+// the compiler would merge the two products into one 32-bit multiply)
+__device__ __forceinline__ uint32_t sr_node_hash(uint32_t own) { return __umul24(own, 0x5BD1E9u); }
 __device__ __forceinline__ SrKey sr_key(uint32_t node_hash, int i) { return {sr_mul(i), node_hash}; }
 __device__ __forceinline__ uint32_t sr_bits(uint32_t b, SrKey k) { return __umul24(b, k.mul) + k.hash; }
 
@@ -156,15 +158,23 @@ struct Store<LBM_STORE_F16> {
   // HIP's default contraction): the exact result is rounded once to FP16,
   // not first to FP32
   __device__ static T round_u(C v) { return (_Float16)v; }
-  // the 13 bits FP32 -> FP16 drops get a random addend, then are cut; a
-  // result in the FP16 subnormal range is rounded to nearest by the
-  // conversion.  The random bits come from the low 24 bits of the pattern,
-  // which scaling by 2^12 leaves alone (mantissa, and the exponent's parity)
+  // gfx950's stochastically rounding conversion (v_cvt_sr_f16_f32) with
+  // the hash as its random word.  Its rule, established on MI355X over
+  // 65,536 (value, word) pairs: with r = the word's top 13 bits, a value
+  // of FP16's normal range is converted as (its bit pattern + r) with the
+  // 13 bits FP16 drops cut (exact; overflow gives Inf, NaN stays NaN); a
+  // value below 2^-14 becomes floor(|v| 2^24 + r / 2^13) units of 2^-24
+  // with v's sign -- stochastic rounding in the subnormal range too.  The
+  // random word hashes the low 24 bits of the pattern, which scaling by
+  // 2^12 leaves alone (mantissa, and the exponent's parity).
   __device__ static T round_sr_u(C v, SrKey key) {
-    const uint32_t b = __float_as_uint(v);
-    const uint32_t r = sr_bits(b, key) >> 19;  // 13 bits
-    // saturating add: a NaN's pattern cannot wrap round to a number
-    return (_Float16)__uint_as_float(__builtin_elementwise_add_sat(b, r) & 0xFFFFE000u);
+    // the conversion writes the low half of its destination and keeps the
+    // high half, which nothing reads: v's own register (dead afterwards)
+    // serves, so no register is cleared for it
+    typedef _Float16 H2 __attribute__((ext_vector_type(2)));
+    const H2 h = __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(H2, v), v,
+                                                  sr_bits(__float_as_uint(v), key), false);
+    return h[0];
   }
 };
 

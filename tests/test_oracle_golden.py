@@ -642,9 +642,14 @@ def test_stochastic_rounding_is_unbiased(storage):
     rng = np.random.default_rng(11)
     n = np.arange(3000, dtype=np.uint64) * 7 + 5  # nodes of a D3Q19 lattice
     own = (n >> np.uint64(6)) * np.uint64(19 * 64) + (n & np.uint64(63))
-    hashes = ((own & np.uint64(0xFFFFFF)) * np.uint64(0x9E3779)) & np.uint64(0xFFFFFFFF)
+    hashes = ((own & np.uint64(0xFFFFFF)) * np.uint64(0x5BD1E9)) & np.uint64(0xFFFFFFFF)
     scale = 4096.0 if storage == O.STORE_F16_SHIFTED else 1.0
-    vals = (rng.choice([-1, 1], 24) * 10 ** rng.uniform(-6, -1, 24)).astype(np.float32)
+    # deviations of 1e-6 .. 1e-1, and (FP16: below 2^-14 once scaled, the
+    # subnormal range, where the conversion rounds stochastically at the
+    # subnormal unit) 1e-12 .. 1e-9
+    vals = np.concatenate([rng.choice([-1, 1], 24) * 10 ** rng.uniform(-6, -1, 24),
+                           rng.choice([-1, 1], 8) * 10 ** rng.uniform(-12, -9, 8)])
+    vals = vals.astype(np.float32)
     for j, v in enumerate(vals):
         v = float(v)
         if storage == O.STORE_F16_SHIFTED:

@@ -186,15 +186,19 @@ struct Store<LBM_STORE_BF16> {
   __device__ static C cvt(T v) { return __uint_as_float(((uint32_t)v) << 16); }
   __device__ static T rest(int, double) { return 0; }
   __device__ static C load(const T* p) { return cvt(gload(p)); }
-  __device__ static T round(C v) {
-    const uint32_t u = __float_as_uint(v);
-    // saturating add, as in round_sr: a NaN pattern near 0x7FFFFFFF or
-    // 0xFFFFFFFF would otherwise wrap round to a signed zero
-    return (uint16_t)(__builtin_elementwise_add_sat(u, 0x7fffu + ((u >> 16) & 1u)) >> 16);
-  }
+  // gfx950's conversions: v_cvt_pk_bf16_f32 rounds to nearest, ties to
+  // even; v_cvt_sr_bf16_f32 rounds stochastically, adding its random word's
+  // top 16 bits to the pattern and cutting the low 16 (established on
+  // MI355X over 65,536 (value, word) pairs, FP32 subnormals included;
+  // overflow gives Inf, NaN stays NaN)
+  __device__ static T round(C v) { return __builtin_bit_cast(uint16_t, (__bf16)v); }
   __device__ static T round_sr(C v, SrKey key) {
-    const uint32_t u = __float_as_uint(v);
-    return (uint16_t)(__builtin_elementwise_add_sat(u, sr_bits(u, key) >> 16) >> 16);
+    // the conversion keeps its destination's high half, which nothing
+    // reads: v's own register (dead afterwards) serves
+    typedef __bf16 B2 __attribute__((ext_vector_type(2)));
+    const B2 h = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(B2, v), v,
+                                                   sr_bits(__float_as_uint(v), key), false);
+    return __builtin_bit_cast(uint16_t, h[0]);
   }
   __device__ static void store(T* p, C v) { gstore(p, round(v)); }
   __device__ static void store_nt(T* p, C v) { gstore_nt(p, round(v)); }

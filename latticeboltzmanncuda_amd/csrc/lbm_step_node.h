@@ -321,8 +321,12 @@ __device__ __forceinline__ void std_collide_node(const StdParams& p,
   // Guo forcing only where the lattice has a body force (uniform per
   // lattice); without one, u = j / rho and the source term is zero
   // a wavefront without a solid node skips the rest-state selection (the
-  // branch is uniform)
-  const bool any_solid = __builtin_amdgcn_ballot_w64(solid) != 0;
+  // branch is uniform).  Only for D2Q9 with 16-bit storage, which is bound
+  // by instruction issue: the second copy of the relaxation costs the 3-D
+  // lattices' batch kernels registers (their eight waves per SIMD then
+  // spill to memory)
+  constexpr bool kSkipSolid = Q == 9 && sizeof(typename St::T) == 2;
+  const bool any_solid = !kSkipSolid || __builtin_amdgcn_ballot_w64(solid) != 0;
   // D2Q9 has no z velocity (its z force is 0: lbm_create checks)
   constexpr bool k2D = L::D == 2;
   if (k.forced) {

@@ -10,8 +10,9 @@
 //      the half-way bounce-back value A_inv(i)(x) instead (+ the moving-wall
 //      term 6 w_i rho_w e_i.u_w when the source lies beyond the moving
 //      face) -- one load per link, the address is selected without a
-//      branch, so all Q loads are in flight together; solid nodes load
-//      nothing;
+//      branch, so all Q loads are in flight together; with FP32 / FP64
+//      storage solid nodes load nothing, with 16-bit storage they gather
+//      like fluid nodes (fewer instructions; see std_gather_pops);
 //   3. collide (BGK + Guo forcing) in FP32 (FP64 for F64 storage);
 //   4. store every direction with all lanes active; solid nodes store the
 //      rest state (w_i, or 0 for shifted storage).  Skipping the stores of
@@ -26,9 +27,13 @@
 //
 // The FP16 / BF16 paths are bound by vector-instruction issue, not by HBM,
 // so the update is written to spend few instructions per node: per-lattice
-// constants come precomputed (StepCoef), addresses need no 32-bit integer
-// multiplications, the body-force terms run only for lattices that have
-// one, and solid lanes get no values.  Large D2Q9 lattices with 16-bit
+// constants come precomputed (StepCoef), in FP16's stored units; addresses
+// need no 32-bit integer multiplications and go through buffer resources
+// (constant offsets in the instructions); the body-force terms run only
+// for lattices that have one; wavefronts skip the moving-wall term, the
+// bounced-link select and (D2Q9) the rest-state select where none of
+// their nodes needs them; 16-bit stores of forced lattices round with the
+// hardware's stochastic conversion.  Large D2Q9 lattices with 16-bit
 // storage take the group step (std_group_step, below): two or four nodes
 // per thread share their loads, stores and address arithmetic, with the
 // same per-node arithmetic, so every width gives the same bytes.

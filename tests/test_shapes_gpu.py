@@ -6,8 +6,10 @@ the size where it is chosen by itself) -- against the one-node step
 oracle.  Shapes cover thin and long z-rows (nz = 4 to 260, tiles of 64
 nodes spanning several rows or a part of one), periodic and walled z, a
 moving wall, a body force, and runs of solid nodes beside fluid rows
-(solid lanes load nothing; their fluid neighbours' bounced links must
-still be right)."""
+(solid lanes load nothing with FP32 storage and gather like fluid lanes
+with 16-bit storage; either way their fluid neighbours' bounced links
+must still be right, and waves with and without solid nodes take
+different code)."""
 import os
 import sys
 

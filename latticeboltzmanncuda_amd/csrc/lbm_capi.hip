@@ -589,6 +589,7 @@ int lbm_create(const lbm_config* cfg, void* stream, lbm_ctx** out) {
   g.n = g.nx * g.ny * g.nz;
   g.dz = make_magic(g.nz);
   g.dy = make_magic(g.ny);
+  g.wrap = wrap_bits(cfg->periodic);
   c->flow.kind = LBM_FLOW_NONE;
   auto bail = [&](int code) {
     lbm_destroy(c);

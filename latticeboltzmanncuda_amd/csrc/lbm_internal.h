@@ -29,7 +29,6 @@ struct TraceRange {
 struct Magic {
   uint32_t mul;
   uint32_t shift;
-  uint32_t d;
   __host__ __device__ uint32_t div(uint32_t n) const {
 #ifdef __HIP_DEVICE_COMPILE__
     const uint32_t t = __umulhi(n, mul);
@@ -44,14 +43,21 @@ inline Magic make_magic(uint32_t d) {
   uint32_t l = 0;
   while ((1ull << l) < d) ++l;
   const uint64_t m = ((1ull << 32) * ((1ull << l) - d)) / d + 1;
-  return Magic{(uint32_t)m, l, d};
+  return Magic{(uint32_t)m, l};
+}
+
+// Periodic axes of a lattice (Grid::wrap bits).
+constexpr uint32_t kWrapX = 1u, kWrapY = 2u, kWrapZ = 4u;
+inline uint32_t wrap_bits(const int periodic[3]) {
+  return (periodic[0] ? kWrapX : 0u) | (periodic[1] ? kWrapY : 0u) | (periodic[2] ? kWrapZ : 0u);
 }
 
 // Geometry of a lattice as the kernels see it.
 struct Grid {
   uint32_t nx, ny, nz;
-  uint32_t n;    // nodes
-  Magic dz, dy;  // division by nz and by ny
+  uint32_t n;     // nodes
+  Magic dz, dy;   // division by nz and by ny
+  uint32_t wrap;  // kWrapX | kWrapY | kWrapZ: the periodic axes
 };
 
 // Per-lattice constants of the standard step in the compute type C,

@@ -249,6 +249,7 @@ int lbm_snapshot_load(lbm_ctx* c, const char* path) {
   }
   c->cfg.lid_face = s.lid_face;
   c->cfg.wall_rho = s.wall_rho;
+  c->grid.wrap = wrap_bits(c->cfg.periodic);
   ++c->params_version;
   c->steps_done = h.steps_done;
   if ((rc = upload(c, c->f[c->cur], pops.data(), pops.size()))) return rc;

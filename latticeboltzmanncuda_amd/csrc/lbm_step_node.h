@@ -111,24 +111,29 @@ __device__ __forceinline__ void std_gather(const Grid& g,
                                            NodeLoad<Q, S>& ld) {
   using L = Lat<Q>;
   const uint32_t c = node_class(cls, n);
-  uint32_t x, y, z;
-  if (Q == 9) {  // D2Q9 lattices have nz == 1
-    z = 0;
-    x = g.dy.div(n);
-    y = n - __umul24(x, g.ny);
-  } else {
-    decode_node(g, n, x, y, z);
-  }
   const uint32_t plane = g.ny * g.nz;
-  // offsets (mod 2^32) from n to the source node n - e for e = -1, 0, +1
-  // along each axis, with periodic wrap (a wall link is flagged in the mask
-  // and its wrapped value is replaced)
-  const uint32_t ox[3] = {x + 1 == g.nx ? plane - g.n : plane, 0u,
-                          x == 0 ? g.n - plane : 0u - plane};
-  const uint32_t oy[3] = {y + 1 == g.ny ? g.nz - plane : g.nz, 0u,
-                          y == 0 ? plane - g.nz : 0u - g.nz};
-  const uint32_t oz[3] = {z + 1 == g.nz ? 1u - g.nz : 1u, 0u,
-                          z == 0 ? g.nz - 1u : 0u - 1u};
+  // FP32 / FP64 storage: offsets (mod 2^32) from n to the source node
+  // n - e for e = -1, 0, +1 along each axis, with periodic wrap along every
+  // axis -- a wall link is flagged in the mask and its wrapped address
+  // replaced -- so that every address formed stays within the lattice
+  // (these loads have no range check)
+  uint32_t ox[3] = {}, oy[3] = {}, oz[3] = {};
+  if constexpr (sizeof(typename Store<S>::T) != 2) {
+    uint32_t x, y, z;
+    if (Q == 9) {  // D2Q9 lattices have nz == 1
+      z = 0;
+      x = g.dy.div(n);
+      y = n - __umul24(x, g.ny);
+    } else {
+      decode_node(g, n, x, y, z);
+    }
+    ox[0] = x + 1 == g.nx ? plane - g.n : plane;
+    ox[2] = x == 0 ? g.n - plane : 0u - plane;
+    oy[0] = y + 1 == g.ny ? g.nz - plane : g.nz;
+    oy[2] = y == 0 ? plane - g.nz : 0u - g.nz;
+    oz[0] = z + 1 == g.nz ? 1u - g.nz : 1u;
+    oz[2] = z == 0 ? g.nz - 1u : 0u - 1u;
+  }
 
   using T = typename Store<S>::T;
   ld.solid = c == kClassSolid;
@@ -158,21 +163,64 @@ __device__ __forceinline__ void std_gather(const Grid& g,
       constexpr uint32_t kRow = (uint32_t)kTile * 2u;  // bytes of one direction's run
       const auto rsrc = buffer_rsrc(A, ld.pop_bytes);
       const uint32_t own_b = 2u * ld.own;
-      // the doubled offsets, selected from doubled (scalar) values
+      // the doubled source indices 2 (n - e_i) mod 2^32
       const uint32_t p2 = 2u * plane, n2 = 2u * g.n, z2 = 2u * g.nz;
-      const uint32_t ox2[3] = {x + 1 == g.nx ? p2 - n2 : p2, 0u, x == 0 ? n2 - p2 : 0u - p2};
-      const uint32_t oy2[3] = {y + 1 == g.ny ? z2 - p2 : z2, 0u, y == 0 ? p2 - z2 : 0u - z2};
-      const uint32_t oz2[3] = {2u * oz[0], 0u, 2u * oz[2]};
+      uint32_t s2[Q];
+      if constexpr (Q == 9) {
+        // D2Q9 (nz == 1; batches and single lattices bound by instruction
+        // issue): one add of a scalar per direction, without wrap.  Along
+        // a walled axis every link that leaves the box is bounced (flagged
+        // in the mask, its pulled address replaced below), and any other
+        // address a lane forms there -- a solid node's -- is read within
+        // the buffer resource's range or as 0, so these serve.  A periodic
+        // axis then wraps the links of nodes on its end faces, in place and
+        // only in wavefronts that hold such a node: sources beyond the
+        // high end (e = -1) move back by the axis's period, those beyond
+        // the low end (e = +1) forward.  The x ends are found by comparing
+        // n with the row length; y is decoded only for lattices periodic
+        // along y.
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+          uint32_t d = 0u;
+          if (L::E[i][0]) d += L::E[i][0] > 0 ? 0u - p2 : p2;
+          if (L::E[i][1]) d += L::E[i][1] > 0 ? 0u - 2u : 2u;
+          s2[i] = 2u * n + d;
+        }
+        auto wrap_axis = [&](int a, bool at_hi, bool at_lo, uint32_t period2) {
+          if (__builtin_amdgcn_ballot_w64(at_hi || at_lo)) {
+            const uint32_t fh = at_hi ? period2 : 0u, fl = at_lo ? period2 : 0u;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+              if (L::E[i][a] < 0) s2[i] -= fh;
+              if (L::E[i][a] > 0) s2[i] += fl;
+            }
+          }
+        };
+        if (g.wrap & kWrapX) wrap_axis(0, n + plane >= g.n, n < plane, n2);
+        if (g.wrap & kWrapY) {
+          const uint32_t x = g.dy.div(n);
+          const uint32_t y = n - __umul24(x, g.ny);  // x, ny < 2^24
+          wrap_axis(1, y + 1 == g.ny, y == 0, p2);
+        }
+      } else {
+        // 3-D lattices (bound by HBM bandwidth; the form above measured
+        // slower for them): doubled offsets per axis, with periodic wrap,
+        // selected from doubled (scalar) values by the decoded coordinates
+        uint32_t x, y, z;
+        decode_node(g, n, x, y, z);
+        const uint32_t ox2[3] = {x + 1 == g.nx ? p2 - n2 : p2, 0u, x == 0 ? n2 - p2 : 0u - p2};
+        const uint32_t oy2[3] = {y + 1 == g.ny ? z2 - p2 : z2, 0u, y == 0 ? p2 - z2 : 0u - z2};
+        const uint32_t oz2[3] = {z + 1 == g.nz ? 2u - z2 : 2u, 0u, z == 0 ? z2 - 2u : 0u - 2u};
+#pragma unroll
+        for (int i = 0; i < Q; ++i)
+          s2[i] = 2u * n + ox2[L::E[i][0] + 1] + oy2[L::E[i][1] + 1] + oz2[L::E[i][2] + 1];
+      }
       // (Q - 1) kRow passed through an empty asm: for D2Q9 it is a power of
       // two, which the compiler would turn into a shift and a mask -- one
       // instruction more than the multiply-add
       uint32_t run_bytes = (uint32_t)(Q - 1) * kRow;
       asm("" : "+s"(run_bytes));
-      auto pull_b = [&](int i) {
-        const uint32_t src2 =
-            2u * n + ox2[L::E[i][0] + 1] + oy2[L::E[i][1] + 1] + oz2[L::E[i][2] + 1];
-        return __umul24(src2 / kRow, run_bytes) + src2;
-      };
+      auto pull_b = [&](int i) { return __umul24(s2[i] / kRow, run_bytes) + s2[i]; };
       // a wavefront without a bounced link among its nodes (a third of
       // them in the batching workload, more in large lattices) skips the
       // per-link selection: the branch is uniform

@@ -165,7 +165,8 @@ __device__ __forceinline__ void std_gather(const Grid& g,
       const uint32_t own_b = 2u * ld.own;
       // the doubled source indices 2 (n - e_i) mod 2^32
       const uint32_t p2 = 2u * plane, n2 = 2u * g.n, z2 = 2u * g.nz;
-      uint32_t s2[Q];
+      uint32_t s2[Q];                                // D2Q9
+      uint32_t ox2[3] = {}, oy2[3] = {}, oz2[3] = {};  // 3-D
       if constexpr (Q == 9) {
         // D2Q9 (nz == 1; batches and single lattices bound by instruction
         // issue): one add of a scalar per direction, without wrap.  Along
@@ -206,21 +207,32 @@ __device__ __forceinline__ void std_gather(const Grid& g,
         // 3-D lattices (bound by HBM bandwidth; the form above measured
         // slower for them): doubled offsets per axis, with periodic wrap,
         // selected from doubled (scalar) values by the decoded coordinates
+        // and summed per direction where its load needs them (formed all
+        // ahead, they hold more registers: the D3Q27 batch kernels spill)
         uint32_t x, y, z;
         decode_node(g, n, x, y, z);
-        const uint32_t ox2[3] = {x + 1 == g.nx ? p2 - n2 : p2, 0u, x == 0 ? n2 - p2 : 0u - p2};
-        const uint32_t oy2[3] = {y + 1 == g.ny ? z2 - p2 : z2, 0u, y == 0 ? p2 - z2 : 0u - z2};
-        const uint32_t oz2[3] = {z + 1 == g.nz ? 2u - z2 : 2u, 0u, z == 0 ? z2 - 2u : 0u - 2u};
-#pragma unroll
-        for (int i = 0; i < Q; ++i)
-          s2[i] = 2u * n + ox2[L::E[i][0] + 1] + oy2[L::E[i][1] + 1] + oz2[L::E[i][2] + 1];
+        ox2[0] = x + 1 == g.nx ? p2 - n2 : p2;
+        ox2[2] = x == 0 ? n2 - p2 : 0u - p2;
+        oy2[0] = y + 1 == g.ny ? z2 - p2 : z2;
+        oy2[2] = y == 0 ? p2 - z2 : 0u - z2;
+        oz2[0] = z + 1 == g.nz ? 2u - z2 : 2u;
+        oz2[2] = z == 0 ? z2 - 2u : 0u - 2u;
       }
+      auto src2 = [&](int i) {
+        if constexpr (Q == 9)
+          return s2[i];
+        else
+          return 2u * n + ox2[L::E[i][0] + 1] + oy2[L::E[i][1] + 1] + oz2[L::E[i][2] + 1];
+      };
       // (Q - 1) kRow passed through an empty asm: for D2Q9 it is a power of
       // two, which the compiler would turn into a shift and a mask -- one
       // instruction more than the multiply-add
       uint32_t run_bytes = (uint32_t)(Q - 1) * kRow;
       asm("" : "+s"(run_bytes));
-      auto pull_b = [&](int i) { return __umul24(s2[i] / kRow, run_bytes) + s2[i]; };
+      auto pull_b = [&](int i) {
+        const uint32_t v = src2(i);
+        return __umul24(v / kRow, run_bytes) + v;
+      };
       // a wavefront without a bounced link among its nodes (a third of
       // them in the batching workload, more in large lattices) skips the
       // per-link selection: the branch is uniform

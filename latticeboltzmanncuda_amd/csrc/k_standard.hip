@@ -104,6 +104,17 @@ __global__ void __launch_bounds__(kBlock)
   mask[n] = m;
 }
 
+// The 16-bit copy of D2Q9 link masks, after the 32-bit ones (layout in
+// lbm_layout.h).
+__global__ void __launch_bounds__(kBlock) k_mask16(Grid g, uint32_t* __restrict__ mask) {
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n) return;
+  const uint32_t m = mask[n];
+  reinterpret_cast<uint16_t*>(mask + g.n)[n] =
+      (uint16_t)((m & (kMask16Lid - 1u)) | ((m & LBM_MASK_LID) ? kMask16Lid : 0u) |
+                 ((m & LBM_MASK_SOLID) ? kMask16Solid : 0u));
+}
+
 // Packs the 2-bit node classes of 16 nodes per word from the link masks.
 __global__ void __launch_bounds__(kBlock)
     k_build_class(Grid g, const uint32_t* __restrict__ mask,
@@ -390,6 +401,12 @@ int launch_build_mask(int q, const Grid& g, const uint8_t* solid,
   LBM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_build_class, dim3(blocks_for((g.n + 15) / 16)),
                      dim3(kBlock), 0, s, g, mask, cls);
+  LBM_HIP(hipGetLastError());
+  return LBM_OK;
+}
+
+int launch_mask16(const Grid& g, uint32_t* mask, hipStream_t s) {
+  hipLaunchKernelGGL(k_mask16, dim3(blocks_for(g.n)), dim3(kBlock), 0, s, g, mask);
   LBM_HIP(hipGetLastError());
   return LBM_OK;
 }

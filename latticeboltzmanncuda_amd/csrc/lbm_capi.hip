@@ -160,7 +160,7 @@ int64_t estimate_ctx_bytes(const lbm_config& cfg) {
   const uint64_t n = (uint64_t)cfg.nx * cfg.ny * cfg.nz;
   int64_t b = 2 * (int64_t)(population_elems(cfg.lattice, n) * storage_bytes(cfg.storage));
   if (cfg.mode == LBM_MODE_STANDARD)
-    b += 4 * (int64_t)n + 4 * (int64_t)((n + 15) / 16);
+    b += (has_mask16(cfg) ? 6 : 4) * (int64_t)n + 4 * (int64_t)((n + 15) / 16);
   else
     b += (int64_t)n;
   b += 4 * 1024 * 8 * 4 + 64;  // reduction scratch, force register
@@ -624,12 +624,14 @@ int lbm_create(const lbm_config* cfg, void* stream, lbm_ctx** out) {
   if ((rc = dev_alloc(c, (void**)&c->red_out, sizeof(double) * 4, "hipMalloc(reduction)")))
     return bail(rc);
   if (cfg->mode == LBM_MODE_STANDARD) {
-    if ((rc = dev_alloc(c, (void**)&c->mask, sizeof(uint32_t) * g.n, "hipMalloc(mask)")))
+    if ((rc = dev_alloc(c, (void**)&c->mask, (has_mask16(*cfg) ? 6 : 4) * (size_t)g.n,
+                        "hipMalloc(mask)")))
       return bail(rc);
     if ((rc = dev_alloc(c, (void**)&c->cls, sizeof(uint32_t) * ((g.n + 15) / 16), "hipMalloc(class map)")))
       return bail(rc);
     if ((rc = launch_build_mask(c->q, g, nullptr, cfg->periodic, cfg->lid_face,
-                                c->mask, c->cls, c->stream)))
+                                c->mask, c->cls, c->stream)) ||
+        (has_mask16(*cfg) && (rc = launch_mask16(g, c->mask, c->stream))))
       return bail(rc);
   } else {
     if ((rc = dev_alloc(c, (void**)&c->flags, g.n, "hipMalloc(flags)")))
@@ -812,7 +814,8 @@ int lbm_upload_solid(lbm_ctx* c, const uint8_t* solid) {
   }
   if ((rc = launch_build_mask(c->q, c->grid, (const uint8_t*)ds.p,
                               c->cfg.periodic, c->cfg.lid_face, c->mask,
-                              c->cls, c->stream)))
+                              c->cls, c->stream)) ||
+      (has_mask16(c->cfg) && (rc = launch_mask16(c->grid, c->mask, c->stream))))
     return rc;
   return sync(c);
 }

@@ -257,7 +257,8 @@ int lbm_snapshot_load(lbm_ctx* c, const char* path) {
                          c->stream));
   if (standard) {
     if ((rc = upload(c, c->mask, mask.data(), mask.size())) ||
-        (rc = upload(c, c->cls, cls.data(), cls.size())))
+        (rc = upload(c, c->cls, cls.data(), cls.size())) ||
+        (has_mask16(c->cfg) && (rc = launch_mask16(c->grid, c->mask, c->stream))))
       return rc;
   } else {
     if ((rc = lbm_upload_flags(c, flags.data()))) return rc;

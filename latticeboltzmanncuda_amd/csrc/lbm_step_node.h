@@ -140,12 +140,15 @@ __device__ __forceinline__ void std_gather(const Grid& g,
   if constexpr (sizeof(T) == 2 && Q == 9) {
     // D2Q9 with 16-bit storage (bound by instruction issue and by the
     // chain of dependent loads before the populations): every lane reads
-    // its link mask, whose top bit marks a solid node, instead of the
-    // class word and then, for boundary nodes, the mask -- one load
-    // latency fewer before the gather, for 4 bytes per node more read
-    const uint32_t mw = gload(mask + n);
-    ld.solid = (mw & LBM_MASK_SOLID) != 0u;
-    ld.m = mw & ~LBM_MASK_SOLID;
+    // its link mask from the 16-bit copy that follows the 32-bit masks
+    // (bit 15 marks a solid node, bit 14 a node next to the moving wall),
+    // instead of the class word and then, for boundary nodes, the mask --
+    // one load latency fewer before the gather, for 2 bytes per node more
+    // read
+    const uint32_t mw =
+        bload16<uint16_t>(buffer_rsrc(mask + g.n, 2u * g.n), 2u * n);
+    ld.solid = (mw & kMask16Solid) != 0u;
+    ld.m = mw & ~kMask16Solid;
   } else {
     ld.solid = c == kClassSolid;
     ld.m = c == kClassBoundary ? gload(mask + n) : 0u;
@@ -364,7 +367,10 @@ __device__ __forceinline__ void std_collide_node(const StdParams& p,
   // only nodes next to the moving face have such links, so a wavefront
   // without one skips the selection (the branch is uniform)
   if (LID) {
-    const uint32_t ml = (m & LBM_MASK_LID) ? m : 0u;  // links of a node next to it
+    // links of a node next to it (the 32-bit mask's flag, or for D2Q9 the
+    // 16-bit copy's, which the 32-bit D2Q9 masks never set)
+    constexpr uint32_t kLidBits = Q == 9 ? LBM_MASK_LID | kMask16Lid : LBM_MASK_LID;
+    const uint32_t ml = (m & kLidBits) ? m : 0u;
     if (__builtin_amdgcn_ballot_w64(ml != 0u) != 0) {
 #pragma unroll
       for (int i = 1; i < Q; ++i) {  // the rest direction never crosses a face

@@ -137,10 +137,9 @@ int launch_build_mask(int q, const Grid& g, const uint8_t* solid,
                       const int periodic[3], int lid_face, uint32_t* mask,
                       uint32_t* cls, hipStream_t s);
 // Whether a standard-mode lattice keeps the 16-bit copy of its link masks
-// (D2Q9 with 16-bit storage), and its refresh from the 32-bit masks.
+// (D2Q9 other than FP64), and its refresh from the 32-bit masks.
 inline bool has_mask16(const lbm_config& cfg) {
-  return cfg.mode == LBM_MODE_STANDARD && cfg.lattice == 9 &&
-         (cfg.storage == LBM_STORE_F16 || cfg.storage == LBM_STORE_BF16);
+  return cfg.mode == LBM_MODE_STANDARD && cfg.lattice == 9 && cfg.storage != LBM_STORE_F64;
 }
 int launch_mask16(const Grid& g, uint32_t* mask, hipStream_t s);
 int launch_init_eq(int q, int storage, const Grid& g, const double* rho,

@@ -46,8 +46,8 @@ constexpr uint32_t kClassInterior = 0;  // fluid, every link streams
 constexpr uint32_t kClassBoundary = 1;  // fluid with bounce-back / lid links
 constexpr uint32_t kClassSolid = 2;     // not updated; holds the rest state
 
-// D2Q9 lattices with 16-bit storage keep a 16-bit copy of their link
-// masks after the 32-bit ones (their mask array holds 6 bytes per node):
+// D2Q9 lattices (other than FP64) keep a 16-bit copy of their link masks
+// after the 32-bit ones (their mask array holds 6 bytes per node):
 // bits 0-13 as in the 32-bit mask, bit 14 next to the moving wall (the
 // 32-bit mask's bit 30), bit 15 solid (bit 31).
 constexpr uint32_t kMask16Lid = 1u << 14, kMask16Solid = 1u << 15;

@@ -4,8 +4,8 @@
 //
 // For node n of lattice g:
 //   1. read its class from the 2-bit class map: interior fluid, boundary
-//      fluid, or solid (D2Q9 with 16-bit storage: its link mask, which
-//      marks solid nodes too);
+//      fluid, or solid (D2Q9 other than FP64: its link mask, which marks
+//      solid nodes too);
 //   2. fluid nodes pull f_i = A_i(x - e_i) with periodic wrap; boundary
 //      nodes first read their link mask and, for each bounced link, load
 //      the half-way bounce-back value A_inv(i)(x) instead (+ the moving-wall
@@ -111,7 +111,7 @@ __device__ __forceinline__ void std_gather(const Grid& g,
                                            const uint32_t* __restrict__ cls, uint32_t n,
                                            NodeLoad<Q, S>& ld) {
   using L = Lat<Q>;
-  const uint32_t c = node_class(cls, n);  // (unused, so not read, by D2Q9 16-bit)
+  const uint32_t c = node_class(cls, n);  // (unused, so not read, by most D2Q9)
   const uint32_t plane = g.ny * g.nz;
   // FP32 / FP64 storage: offsets (mod 2^32) from n to the source node
   // n - e for e = -1, 0, +1 along each axis, with periodic wrap along every
@@ -137,14 +137,13 @@ __device__ __forceinline__ void std_gather(const Grid& g,
   }
 
   using T = typename Store<S>::T;
-  if constexpr (sizeof(T) == 2 && Q == 9) {
-    // D2Q9 with 16-bit storage (bound by instruction issue and by the
-    // chain of dependent loads before the populations): every lane reads
-    // its link mask from the 16-bit copy that follows the 32-bit masks
-    // (bit 15 marks a solid node, bit 14 a node next to the moving wall),
-    // instead of the class word and then, for boundary nodes, the mask --
-    // one load latency fewer before the gather, for 2 bytes per node more
-    // read
+  if constexpr (sizeof(T) != 8 && Q == 9) {
+    // D2Q9 with FP32 or 16-bit storage: every lane reads its link mask
+    // from the 16-bit copy that follows the 32-bit masks (bit 15 marks a
+    // solid node, bit 14 a node next to the moving wall), instead of the
+    // class word and then, for boundary nodes, the mask -- one load
+    // latency fewer before the gather, for under 2 bytes per node more
+    // read (C4 batches: FP16 3.5 %, FP32 2.4 % faster)
     const uint32_t mw =
         bload16<uint16_t>(buffer_rsrc(mask + g.n, 2u * g.n), 2u * n);
     ld.solid = (mw & kMask16Solid) != 0u;

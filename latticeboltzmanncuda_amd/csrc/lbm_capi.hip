@@ -589,7 +589,7 @@ int lbm_create(const lbm_config* cfg, void* stream, lbm_ctx** out) {
   g.n = g.nx * g.ny * g.nz;
   g.dz = make_magic(g.nz);
   g.dy = make_magic(g.ny);
-  g.wrap = wrap_bits(cfg->periodic);
+  g.flags = wrap_bits(cfg->periodic);
   c->flow.kind = LBM_FLOW_NONE;
   auto bail = [&](int code) {
     lbm_destroy(c);
@@ -808,10 +808,14 @@ int lbm_upload_solid(lbm_ctx* c, const uint8_t* solid) {
   if (int rc = need_mode(c, LBM_MODE_STANDARD, "lbm_upload_solid")) return rc;
   DevBuf ds;
   int rc;
+  bool any_solid = false;
   if (solid) {
     if ((rc = ds.alloc(c->grid.n))) return rc;
     if ((rc = h2d(c, ds.p, solid, c->grid.n))) return rc;
+    for (uint32_t n = 0; n < c->grid.n && !any_solid; ++n) any_solid = solid[n] != 0;
   }
+  c->grid.flags = (c->grid.flags & ~kReadMasks) | (any_solid ? kReadMasks : 0u);
+  ++c->params_version;
   if ((rc = launch_build_mask(c->q, c->grid, (const uint8_t*)ds.p,
                               c->cfg.periodic, c->cfg.lid_face, c->mask,
                               c->cls, c->stream)) ||

@@ -46,8 +46,10 @@ inline Magic make_magic(uint32_t d) {
   return Magic{(uint32_t)m, l};
 }
 
-// Periodic axes of a lattice (Grid::wrap bits).
-constexpr uint32_t kWrapX = 1u, kWrapY = 2u, kWrapZ = 4u;
+// Grid::flags: the periodic axes, and whether the 3-D standard step reads
+// every node's link mask directly (lattices with solid nodes) instead of
+// its 2-bit class first (the mask only for boundary nodes).
+constexpr uint32_t kWrapX = 1u, kWrapY = 2u, kWrapZ = 4u, kReadMasks = 8u;
 inline uint32_t wrap_bits(const int periodic[3]) {
   return (periodic[0] ? kWrapX : 0u) | (periodic[1] ? kWrapY : 0u) | (periodic[2] ? kWrapZ : 0u);
 }
@@ -57,7 +59,7 @@ struct Grid {
   uint32_t nx, ny, nz;
   uint32_t n;     // nodes
   Magic dz, dy;   // division by nz and by ny
-  uint32_t wrap;  // kWrapX | kWrapY | kWrapZ: the periodic axes
+  uint32_t flags;  // kWrapX | kWrapY | kWrapZ (periodic axes) | kReadMasks
 };
 
 // Per-lattice constants of the standard step in the compute type C,

@@ -249,7 +249,13 @@ int lbm_snapshot_load(lbm_ctx* c, const char* path) {
   }
   c->cfg.lid_face = s.lid_face;
   c->cfg.wall_rho = s.wall_rho;
-  c->grid.wrap = wrap_bits(c->cfg.periodic);
+  bool any_solid = false;
+  for (size_t n = 0; standard && n < N && !any_solid; ++n) {
+    uint32_t m;
+    std::memcpy(&m, mask.data() + 4 * n, 4);
+    any_solid = (m & LBM_MASK_SOLID) != 0u;
+  }
+  c->grid.flags = wrap_bits(c->cfg.periodic) | (any_solid ? kReadMasks : 0u);
   ++c->params_version;
   c->steps_done = h.steps_done;
   if ((rc = upload(c, c->f[c->cur], pops.data(), pops.size()))) return rc;

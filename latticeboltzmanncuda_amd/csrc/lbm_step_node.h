@@ -4,8 +4,8 @@
 //
 // For node n of lattice g:
 //   1. read its class from the 2-bit class map: interior fluid, boundary
-//      fluid, or solid (D2Q9 other than FP64: its link mask, which marks
-//      solid nodes too);
+//      fluid, or solid (D2Q9 other than FP64, and 3-D lattices with solid
+//      nodes: its link mask, which marks solid nodes too);
 //   2. fluid nodes pull f_i = A_i(x - e_i) with periodic wrap; boundary
 //      nodes first read their link mask and, for each bounced link, load
 //      the half-way bounce-back value A_inv(i)(x) instead (+ the moving-wall
@@ -148,6 +148,17 @@ __device__ __forceinline__ void std_gather(const Grid& g,
         bload16<uint16_t>(buffer_rsrc(mask + g.n, 2u * g.n), 2u * n);
     ld.solid = (mw & kMask16Solid) != 0u;
     ld.m = mw & ~kMask16Solid;
+  } else if (sizeof(T) != 8 && (g.flags & kReadMasks)) {
+    // 3-D lattices with solid nodes (many boundary nodes): every lane
+    // reads its link mask, whose top bit marks a solid node -- one load
+    // latency fewer before the gather, for 3-4 bytes per node more read
+    // (D3Q19 FP32 channel with 20 % spheres 1.2 % faster, D3Q19 FP16
+    // batches of sphere channels 1-3 %); lattices without solids (few or
+    // no boundary nodes: the Taylor-Green vortex 2 % faster this way)
+    // read the 2-bit class and the mask only where needed
+    const uint32_t mw = gload(mask + n);
+    ld.solid = (mw & LBM_MASK_SOLID) != 0u;
+    ld.m = mw & ~LBM_MASK_SOLID;
   } else {
     ld.solid = c == kClassSolid;
     ld.m = c == kClassBoundary ? gload(mask + n) : 0u;
@@ -211,8 +222,8 @@ __device__ __forceinline__ void std_gather(const Grid& g,
             }
           }
         };
-        if (g.wrap & kWrapX) wrap_axis(0, n + plane >= g.n, n < plane, n2);
-        if (g.wrap & kWrapY) {
+        if (g.flags & kWrapX) wrap_axis(0, n + plane >= g.n, n < plane, n2);
+        if (g.flags & kWrapY) {
           const uint32_t x = g.dy.div(n);
           const uint32_t y = n - __umul24(x, g.ny);  // x, ny < 2^24
           wrap_axis(1, y + 1 == g.ny, y == 0, p2);

@@ -1,11 +1,13 @@
-"""Standard-mode steps on unusual shapes -- FP32 D3Q19 / D3Q27, and D2Q9
+"""Standard-mode steps on unusual shapes -- FP32 D3Q19 / D3Q27, 16-bit
+D3Q19 with and without solid nodes, FP32 D2Q9, and D2Q9
 with 16-bit storage run by the group step (four nodes per thread where ny
 is a multiple of 4, two where it is even; LBM_GROUP_STEP=1 selects it below
 the size where it is chosen by itself) -- against the one-node step
 (LBM_GROUP_STEP=0) and the one-lattice batch (same bytes), and against the
 oracle.  Shapes cover thin and long z-rows (nz = 4 to 260, tiles of 64
 nodes spanning several rows or a part of one), periodic and walled z, a
-moving wall, a body force, and runs of solid nodes beside fluid rows
+moving wall, a body force, periodic axes in every combination the wrap
+code distinguishes, and runs of solid nodes beside fluid rows
 (solid lanes load nothing with FP32 storage and gather like fluid lanes
 with 16-bit storage; either way their fluid neighbours' bounced links
 must still be right, and waves with and without solid nodes take
@@ -66,6 +68,18 @@ CASES = [
     (9, (12, 8, 1), (0, 0, 0), None, True, True, F16),
     (9, (20, 132, 1), (0, 0, 0), 0, True, False, BF16),
     (9, (8, 64, 1), (1, 0, 0), 1, False, True, BF16),
+    # periodic along y only (the wrap decodes y; walled x ends pull
+    # unwrapped addresses that the bounce replaces), odd and even ny
+    (9, (20, 34, 1), (0, 1, 0), 0, False, True, F16),
+    (9, (18, 25, 1), (0, 1, 0), None, False, True, BF16),
+    # FP32 D2Q9 (its step reads the 16-bit link masks too)
+    (9, (22, 30, 1), (1, 0, 0), 1, True, True, F32),
+    (9, (16, 20, 1), (0, 1, 0), 0, False, True, F32),
+    # 3-D 16-bit lattices with solid nodes (every node reads its link
+    # mask) and without (class map first)
+    (19, (6, 5, 70), (1, 0, 0), 1, False, True, F16),
+    (19, (7, 4, 66), (1, 0, 1), 0, True, False, BF16),
+    (19, (5, 6, 40), (1, 1, 1), None, False, True, F16),
 ]
 
 

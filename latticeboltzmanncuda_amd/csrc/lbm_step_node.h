@@ -111,7 +111,8 @@ __device__ __forceinline__ void std_gather(const Grid& g,
                                            const uint32_t* __restrict__ cls, uint32_t n,
                                            NodeLoad<Q, S>& ld) {
   using L = Lat<Q>;
-  const uint32_t c = node_class(cls, n);  // (unused, so not read, by most D2Q9)
+  // the node's 2-bit class (read only on the branch below that uses it)
+  const uint32_t c = node_class(cls, n);
   const uint32_t plane = g.ny * g.nz;
   // FP32 / FP64 storage: offsets (mod 2^32) from n to the source node
   // n - e for e = -1, 0, +1 along each axis, with periodic wrap along every

@@ -80,6 +80,12 @@ CASES = [
     (19, (6, 5, 70), (1, 0, 0), 1, False, True, F16),
     (19, (7, 4, 66), (1, 0, 1), 0, True, False, BF16),
     (19, (5, 6, 40), (1, 1, 1), None, False, True, F16),
+    # axes of one node: periodic (the node is its own neighbour) and
+    # walled (every link along the axis bounces)
+    (9, (1, 40, 1), (1, 1, 0), None, False, True, F16),
+    (9, (7, 1, 1), (1, 0, 0), None, False, True, F16),
+    (19, (1, 1, 70), (1, 1, 1), None, False, True, F32),
+    (19, (2, 1, 9), (0, 1, 0), 0, False, True, BF16),
 ]
 
 

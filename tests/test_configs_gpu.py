@@ -220,9 +220,8 @@ def test_c4_512_lattices_batched_equal_alone(storage):
             c.step(steps)
             alone = c.populations()
         assert np.array_equal(batched[k], alone), k
-    if storage != capi.STORE_F32:
-        return
-    # sampled members against the oracle: the first cavity and channel and
+    # sampled members against the oracle (FP32 storage; FP16 against its
+    # emulation of the FP16 store path): the first cavity and channel and
     # the two largest lattices
     kinds = [i["kind"] for _, _, i in specs]
     sizes = [c.nx * c.ny for c, _, _ in specs]
@@ -239,12 +238,17 @@ def test_c4_512_lattices_batched_equal_alone(storage):
         p = O.std_params(tau=cfg.tau, force=tuple(cfg.force),
                          lid_face=cfg.lid_face, lid_u=tuple(cfg.lid_u),
                          periodic=per)
+        f16 = storage == capi.STORE_F16
         fo = O.std_run_f32(9, dims, O.std_init_eq(9, dims), mask, p, steps,
-                           O.STORE_F32_SHIFTED)
+                           O.STORE_F16_SHIFTED if f16 else O.STORE_F32_SHIFTED)
         e = rel_err(batched[k], fo)
         say(f"  lattice {k} {info['kind']} {dims[:2]} tau {cfg.tau:.4f}: "
             f"rel err {e:.2e}")
-        assert e <= 1e-6
+        # FP16: the same store rule; where the two FP32 collisions straddle
+        # a rounding boundary a value lands one FP16 step apart and the
+        # flow carries it on (measured after 100 steps: channels 7e-8 to
+        # 1.2e-6, low-tau cavities up to 3e-4)
+        assert e <= (1e-3 if f16 else 1e-6)
 
 
 # ------------------------------------------------------------------ C5

@@ -306,3 +306,25 @@ def test_c5_served_results_equal_alone_and_oracle():
             f"of max |u| = {np.abs(uo).max():.2e})")
         assert out["steps"] == steps
         assert e <= TOL
+    # the same four requests as served above in FP16 among the 64, against
+    # the oracle's emulation of the FP16 store path (forced lattices: the
+    # same stochastic rounding under the same node hash)
+    for k in pick:
+        r, out = reqs[k], res[k]
+        cfg = r["cfg"]
+        dims = (cfg.nx, cfg.ny, cfg.nz)
+        per = tuple(cfg.periodic)
+        mask = O.std_build_mask(cfg.lattice, dims, r["solid"], per)
+        p = O.std_params(tau=cfg.tau, force=tuple(cfg.force), periodic=per)
+        fo = O.std_run_f32(cfg.lattice, dims,
+                           O.std_init_eq(cfg.lattice, dims), mask, p,
+                           out["steps"], O.STORE_F16_SHIFTED)
+        _, uo = O.std_moments(cfg.lattice, dims, fo, mask, tuple(cfg.force))
+        e = vel_err(out["u"], uo)
+        say(f"  FP16 D{'2Q9' if cfg.lattice == 9 else '3Q19'} {dims}: u err "
+            f"{e:.2e} against the FP16 emulation")
+        # measured 7e-6 to 3.8e-5 (rounding flips where the two FP32
+        # collisions straddle a boundary); the emulation's own FP16 and FP32
+        # runs of these requests differ by 4.8e-4 to 9.2e-4, so another
+        # store or rounding rule fails this
+        assert e <= 2e-4

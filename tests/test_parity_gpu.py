@@ -308,6 +308,64 @@ def test_reference_staged_equals_fused_full_step(data_dir, dims, spacing,
     assert rel_err(outs[1][1], outs[0][1]) <= 1e-6
 
 
+def test_reference_long_reflection_groups_vs_staged_and_oracle():
+    # Records that send many of their 80 slots into a few directions: the
+    # (receiver, direction) groups of the fused step's gather hold up to 42
+    # entries (the meshes' at most 10), so the gather, which takes a group's
+    # entries four at a time, runs up to 11 rounds in one wavefront while
+    # the other lanes of it are done.  Reflecting nodes are at least 3 nodes
+    # apart, so no target is flagged; one sits on the x = 0 face, so some
+    # of its slots' sources lie outside the box.
+    dims, tau, steps = (12, 10, 9), 0.7, 20
+    say(f"case: reference long reflection groups {dims} x {steps}")
+    rng = np.random.default_rng(7)
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    where = [(0, 4, 4), (3, 1, 1), (4, 6, 5), (8, 2, 6), (8, 7, 2)]
+    node = np.array([idx(x, y, z, dims) for x, y, z in where], np.int32)
+    flags = np.zeros(n, np.uint8)
+    flags[node] = O.FLAG_INTERNAL | O.FLAG_REFLECTED
+    dirs = np.zeros((len(node), 14), np.uint32)
+    longest = 0
+    for r in range(len(node)):
+        main = rng.choice(np.arange(1, 27), 3, replace=False)
+        d = np.concatenate([np.full(40, main[0]), np.full(17, main[1]),
+                            np.full(5, main[2]),
+                            rng.integers(0, 27, 80 - 62)])
+        rng.shuffle(d)
+        for s in range(1, 81):
+            dirs[r, s // 6] |= np.uint32(int(d[s - 1]) << (5 * (s % 6)))
+        longest = max(longest, int(np.bincount(d[d > 0]).max()))
+    assert longest >= 40
+    w = rng.uniform(0.0, 0.02, (len(node), 81)).astype(np.float32)
+    recs = (node, dirs.reshape(-1), w.reshape(-1))
+    f = (0.001 * (1 + 0.1 * rng.standard_normal(27 * n))).astype(np.float32)
+    outs = []
+    for staged in (True, False):
+        with ref_ctx(dims, tau=tau) as c:
+            c.set_populations(f)
+            c.upload_flags(flags)
+            c.upload_reflections(*recs)
+            c.set_force([0, 0, 0])
+            for _ in range(steps):
+                if staged:
+                    c.stream()
+                    c.reflect(True)
+                    c.collide(tau)
+                else:
+                    c.step_reference(1, True)
+            outs.append((c.populations(), c.force()))
+    Fo = np.zeros(3, np.float32)
+    Fo64 = np.zeros(3, np.float64)
+    o = O.ref_run_main(dims, f, flags, recs, tau, O.FLOW_NONE, [0, 0],
+                       np.zeros(27, np.float32), Fo, steps, Fo64)
+    say(f"  force staged {outs[0][1]} fused {outs[1][1]} exact {Fo64}")
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[1][0], o)
+    assert rel_err(outs[1][1], outs[0][1]) <= 1e-6
+    assert rel_err(outs[1][1], Fo64) <= TOL
+
+
 # ------------------------------------------------ standard mode vs oracle
 def _std_ctx(q, dims, storage=capi.STORE_F32, **kw):
     return capi.Context(capi.make_config(*dims, lattice=q, storage=storage,

@@ -343,8 +343,11 @@ constexpr SlotTerms make_slot_terms() {
 }
 __constant__ constexpr SlotTerms kSlotTerms = make_slot_terms();
 
-// Gather pass of the fused reference step: one thread per group (t, j).
-// The group's value is the staged pipeline's f_j(t) after reflection:
+// Gather pass of the fused reference step: one thread per group (t, j),
+// the threads ordered by reflecting node x, then j (groups of one x read
+// the same neighbours of x, so a wavefront touches fewer cache lines), each
+// writing its sum to its group's slot gid of gsum.  The group's value is
+// the staged pipeline's f_j(t) after reflection:
 // the streamed value A_j(x) of the reflecting node x = t - e_j, then, in
 // ascending slot order, + W[s] * F_k(x) with F_k(x) = A_k(x - e_k) (0
 // outside the box) -- the same additions in the same order as the staged
@@ -355,7 +358,8 @@ __constant__ constexpr SlotTerms kSlotTerms = make_slot_terms();
 __global__ void __launch_bounds__(kBlock)
     k_ref_reflect_gather(const float* __restrict__ A, uint32_t ngrp,
                          const uint32_t* __restrict__ gsrc,
-                         const uint32_t* __restrict__ gbeg, uint32_t nent,
+                         const uint32_t* __restrict__ gbeg,
+                         const uint32_t* __restrict__ gid, uint32_t nent,
                          const uint32_t* __restrict__ eidx,
                          const float* __restrict__ ew,
                          const uint8_t* __restrict__ es, float* __restrict__ gsum,
@@ -365,6 +369,7 @@ __global__ void __launch_bounds__(kBlock)
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   if (gi < ngrp) {
     const uint32_t src = gsrc[gi];
+    const uint32_t grp = gid[gi];
     float acc = A[src];
     const uint32_t end = gi + 1 < ngrp ? gbeg[gi + 1] : nent;
     // the group's direction j: gsrc is the element index of (x, j)
@@ -395,7 +400,7 @@ __global__ void __launch_bounds__(kBlock)
         }
       }
     }
-    gsum[gi] = acc;
+    gsum[grp] = acc;
   }
   if (partials) {
     const float sx = wave_sum(fx), sy = wave_sum(fy), sz = wave_sum(fz);
@@ -564,7 +569,7 @@ int launch_ref_step_full(const Grid& g, const float* A, float* B, float tau,
                          const FlowSpec& flow, const uint32_t* cls,
                          const uint32_t* inmask, const int32_t* rxbase,
                          int64_t ngrp, const uint32_t* gsrc, const uint32_t* gbeg,
-                         float* gsum, int64_t nent, const uint32_t* eidx,
+                         const uint32_t* gid, float* gsum, int64_t nent, const uint32_t* eidx,
                          const float* ew, const uint8_t* es, float* partials,
                          float* force, int rev, hipStream_t s) {
   const unsigned nb = blocks_for(g.n);
@@ -572,7 +577,7 @@ int launch_ref_step_full(const Grid& g, const float* A, float* B, float tau,
   const bool with_force = cls && ngrp > 0 && force && partials;
   if (cls && ngrp > 0) {
     hipLaunchKernelGGL(k_ref_reflect_gather, dim3(blocks_for((uint32_t)ngrp)), dim3(kBlock), 0, s,
-                       A, (uint32_t)ngrp, gsrc, gbeg, (uint32_t)nent, eidx, ew, es, gsum,
+                       A, (uint32_t)ngrp, gsrc, gbeg, gid, (uint32_t)nent, eidx, ew, es, gsum,
                        with_force ? partials : nullptr);
     LBM_HIP(hipGetLastError());
   }

@@ -301,6 +301,7 @@ int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
   dev_free(c, (void**)&c->ref_rxbase, sizeof(int32_t) * (size_t)N);
   dev_free(c, (void**)&c->ref_gsrc, sizeof(uint32_t) * ng);
   dev_free(c, (void**)&c->ref_gbeg, sizeof(uint32_t) * ng);
+  dev_free(c, (void**)&c->ref_gid, sizeof(uint32_t) * ng);
   dev_free(c, (void**)&c->ref_gsum, sizeof(float) * ng);
   dev_free(c, (void**)&c->ref_eidx, sizeof(uint32_t) * ne);
   dev_free(c, (void**)&c->ref_ew, sizeof(float) * ne);
@@ -371,6 +372,44 @@ int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
     }
   }
   if (eidx.size() >= (1ull << 31)) return fail(LBM_ERR_UNSUPPORTED, "too many reflected links");
+  // The gather's threads take the groups by reflecting node x, then
+  // direction (gsrc = pidx(x, d) gives both): the groups of one x read the
+  // same neighbours of x (up to three slots share an incoming direction),
+  // so a wavefront touches fewer cache lines than in receiver order (on
+  // the 128^3 angled cube 193 k lines over all wavefronts against 295 k).
+  // Each thread writes its sum to its group's slot; the tables follow the
+  // thread order.
+  const size_t ngs = gsrc.size();
+  std::vector<uint32_t> gid(ngs);
+  for (size_t q = 0; q < ngs; ++q) gid[q] = (uint32_t)q;
+  auto xd = [&](uint32_t q) {
+    const uint32_t e = gsrc[q];
+    const uint64_t x = (uint64_t)(e / (27u * kTile)) * kTile + e % kTile;
+    return x * 32u + (e / kTile) % 27u;
+  };
+  std::stable_sort(gid.begin(), gid.end(), [&](uint32_t a, uint32_t b) { return xd(a) < xd(b); });
+  {
+    std::vector<uint32_t> tsrc(ngs), tbeg(ngs), te(eidx.size());
+    std::vector<float> tw(ew.size());
+    std::vector<uint8_t> ts(es.size());
+    size_t pos = 0;
+    for (size_t i = 0; i < ngs; ++i) {
+      const uint32_t q = gid[i];
+      const size_t b = gbeg[q], e = q + 1 < ngs ? gbeg[q + 1] : eidx.size();
+      tsrc[i] = gsrc[q];
+      tbeg[i] = (uint32_t)pos;
+      for (size_t k = b; k < e; ++k, ++pos) {
+        te[pos] = eidx[k];
+        tw[pos] = ew[k];
+        ts[pos] = es[k];
+      }
+    }
+    gsrc.swap(tsrc);
+    gbeg.swap(tbeg);
+    eidx.swap(te);
+    ew.swap(tw);
+    es.swap(ts);
+  }
   c->ref_ngrp = (int64_t)gsrc.size();
   c->ref_nent = (int64_t)eidx.size();
   c->ref_partials_cap = ref_gather_partials(c->ref_ngrp);
@@ -381,6 +420,7 @@ int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
       (rc = dev_alloc(c, (void**)&c->ref_rxbase, sizeof(int32_t) * (size_t)N, "hipMalloc(ref receivers)")) ||
       (rc = dev_alloc(c, (void**)&c->ref_gsrc, sizeof(uint32_t) * ng2, "hipMalloc(ref groups)")) ||
       (rc = dev_alloc(c, (void**)&c->ref_gbeg, sizeof(uint32_t) * ng2, "hipMalloc(ref groups)")) ||
+      (rc = dev_alloc(c, (void**)&c->ref_gid, sizeof(uint32_t) * ng2, "hipMalloc(ref groups)")) ||
       (rc = dev_alloc(c, (void**)&c->ref_gsum, sizeof(float) * ng2, "hipMalloc(ref group sums)")) ||
       (rc = dev_alloc(c, (void**)&c->ref_eidx, sizeof(uint32_t) * ne2, "hipMalloc(ref entries)")) ||
       (rc = dev_alloc(c, (void**)&c->ref_ew, sizeof(float) * ne2, "hipMalloc(ref weights)")) ||
@@ -395,6 +435,7 @@ int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
   if (!gsrc.empty() &&
       ((rc = h2d(c, c->ref_gsrc, gsrc.data(), sizeof(uint32_t) * gsrc.size())) ||
        (rc = h2d(c, c->ref_gbeg, gbeg.data(), sizeof(uint32_t) * gbeg.size())) ||
+       (rc = h2d(c, c->ref_gid, gid.data(), sizeof(uint32_t) * gid.size())) ||
        (rc = h2d(c, c->ref_eidx, eidx.data(), sizeof(uint32_t) * eidx.size())) ||
        (rc = h2d(c, c->ref_ew, ew.data(), sizeof(float) * ew.size())) ||
        (rc = h2d(c, c->ref_es, es.data(), sizeof(uint8_t) * es.size()))))
@@ -495,7 +536,7 @@ int step_reference_full(lbm_ctx* c, int n, int with_force) {
     if ((rc = launch_ref_step_full(c->grid, (const float*)c->f[c->cur], (float*)c->f[c->cur ^ 1],
                                    c->cfg.tau, c->flow, geom ? c->ref_cls : nullptr,
                                    c->ref_inmask, c->ref_rxbase, c->ref_ngrp, c->ref_gsrc,
-                                   c->ref_gbeg, c->ref_gsum, c->ref_nent, c->ref_eidx, c->ref_ew,
+                                   c->ref_gbeg, c->ref_gid, c->ref_gsum, c->ref_nent, c->ref_eidx, c->ref_ew,
                                    c->ref_es, geom && with_force ? c->ref_partials : nullptr,
                                    geom && with_force ? c->force : nullptr, c->cur, c->stream)))
       return rc;
@@ -654,7 +695,7 @@ int lbm_destroy(lbm_ctx* c) {
                   (void*)c->flow_mask, (void*)c->force,
                   (void*)c->force_partials, (void*)c->red, (void*)c->red_out,
                   (void*)c->ref_cls, (void*)c->ref_inmask, (void*)c->ref_rxbase,
-                  (void*)c->ref_gsrc, (void*)c->ref_gbeg, (void*)c->ref_gsum,
+                  (void*)c->ref_gsrc, (void*)c->ref_gbeg, (void*)c->ref_gid, (void*)c->ref_gsum,
                   (void*)c->ref_eidx, (void*)c->ref_ew, (void*)c->ref_es,
                   (void*)c->ref_partials, c->stage})
     if (p) (void)hipFree(p);

@@ -177,7 +177,7 @@ int launch_ref_step_full(const Grid& g, const float* A, float* B, float tau,
                          const FlowSpec& flow, const uint32_t* cls,
                          const uint32_t* inmask, const int32_t* rxbase,
                          int64_t ngrp, const uint32_t* gsrc, const uint32_t* gbeg,
-                         float* gsum, int64_t nent, const uint32_t* eidx,
+                         const uint32_t* gid, float* gsum, int64_t nent, const uint32_t* eidx,
                          const float* ew, const uint8_t* es, float* partials,
                          float* force, int rev, hipStream_t s);
 int ref_gather_partials(int64_t ngrp);
@@ -247,8 +247,10 @@ struct lbm_ctx {
   uint32_t* ref_inmask = nullptr;  // per node: bit j = a reflected link arrives in direction j
   int32_t* ref_rxbase = nullptr;   // per node: first group of a receiver (groups ascend in j)
   int64_t ref_ngrp = 0;            // groups (t, j)
-  uint32_t* ref_gsrc = nullptr;    // per group: element index of A_j(x)
-  uint32_t* ref_gbeg = nullptr;    // per group: first entry; the last group ends at ref_nent
+  // the gather's threads take the groups by reflecting node, then direction
+  uint32_t* ref_gsrc = nullptr;    // per thread: element index of its group's A_j(x)
+  uint32_t* ref_gbeg = nullptr;    // per thread: first entry; the last ends at ref_nent
+  uint32_t* ref_gid = nullptr;     // per thread: its group (slot of gsum)
   float* ref_gsum = nullptr;       // per group: the gathered f_j(t) of the current step
   int64_t ref_nent = 0;            // entries (one per reflected slot)
   uint32_t* ref_eidx = nullptr;    // per entry: element index of F_k(x) = A_k(x - e_k), ~0u outside

@@ -89,6 +89,44 @@ __device__ __forceinline__ void pull27(const Grid& g, const float* __restrict__ 
   }
 }
 
+// Byte offsets of the linear pull (lattices whose nz is a multiple of 64,
+// so that n, n +- nz and n +- ny nz sit at the same place in their tiles):
+// b[j] = 4 (pidx(n - e_j, j) - pidx(n, 0)) for a node whose z-source stays
+// in its tile, computed on the host.
+struct PullOffsets {
+  int32_t b[27];
+};
+
+// pull27 for such lattices through a buffer resource over the population
+// array: the source of direction j is the node's own offset plus b[j], plus
+// one tile crossing for a z-source beyond the tile's first or last node,
+// and a source outside the box gets an offset past the array, which reads
+// 0.  Two instructions per direction instead of the index arithmetic.
+// zero: every direction reads 0 (an internal node, zeroed by the
+// reflection stage) without touching memory.
+__device__ __forceinline__ void pull27_lin(const Grid& g, __amdgpu_buffer_rsrc_t r,
+                                           const PullOffsets& po, uint32_t n, int x, int y,
+                                           int z, bool zero, float f[27]) {
+  // a source one node up z from a tile's last node is the next tile's
+  // first (e_z = -1), one node down from its first node the previous
+  // tile's last (e_z = +1): the linear offset misses by 26 rows of 64
+  constexpr uint32_t kCross = (27u - 1u) * kTile * 4u;
+  const uint32_t own = pidx<27>(n, 0) * 4u;
+  const uint32_t l = n & (kTile - 1);
+  const uint32_t base[3] = {own + (l == kTile - 1 ? kCross : 0u), own,
+                            own - (l == 0 ? kCross : 0u)};
+  const bool bx[3] = {x + 1 < (int)g.nx, true, x > 0};
+  const bool by[3] = {y + 1 < (int)g.ny, true, y > 0};
+  const bool bz[3] = {z + 1 < (int)g.nz, true, z > 0};
+#pragma unroll
+  for (int j = 0; j < 27; ++j) {
+    const int a = L27::E[j][0] + 1, b = L27::E[j][1] + 1, c = L27::E[j][2] + 1;
+    const bool in = !zero && bx[a] && by[b] && bz[c];
+    const uint32_t off = in ? base[c] + (uint32_t)po.b[j] : 0x80000000u;
+    f[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+  }
+}
+
 // The relaxation time and its reciprocal rounded to double (computed on
 // the host); see div_tau.
 struct Relax {
@@ -420,17 +458,32 @@ __global__ void __launch_bounds__(kBlock)
 //      its in-mask; its groups ascend in j from rxbase[n]);
 //   C  collide27; F flow injection.
 // The populations are bit-identical to the staged passes.
-template <bool GEOM, bool NT>
+template <bool GEOM, bool NT, bool LIN>
 __device__ __forceinline__ void step_full_node(
     const Grid& g, const float* __restrict__ A, float* __restrict__ B, const Relax& tau,
     const FlowSpec& flow, const uint32_t* __restrict__ cls, const uint32_t* __restrict__ inmask,
-    const int32_t* __restrict__ rxbase, const float* __restrict__ gsum, uint32_t n) {
+    const int32_t* __restrict__ rxbase, const float* __restrict__ gsum, const PullOffsets& po,
+    uint32_t n) {
   uint32_t ux, uy, uz;
   decode(g, n, ux, uy, uz);
   const int x = (int)ux, y = (int)uy, z = (int)uz;
   float f[27];
-  pull27(g, A, n, x, y, z, f);
-  if (GEOM) {
+  if (LIN) {
+    // the class word first: an internal node's loads read 0 without
+    // touching memory, which is its zeroing
+    const uint32_t c = GEOM ? (cls[n >> 4] >> (2u * (n & 15u))) & 3u : 0u;
+    pull27_lin(g, buffer_rsrc(A, 27u * 4u * g.n), po, n, x, y, z, c == 2u, f);
+    if (c == 1u) {
+      const uint32_t m = inmask[n];
+      uint32_t p = (uint32_t)rxbase[n];
+#pragma unroll
+      for (int j = 1; j < 27; ++j)
+        if ((m >> j) & 1u) f[j] = gsum[p++];
+    }
+  } else {
+    pull27(g, A, n, x, y, z, f);
+  }
+  if (GEOM && !LIN) {
     const uint32_t c = (cls[n >> 4] >> (2u * (n & 15u))) & 3u;
     if (c == 2u) {
 #pragma unroll
@@ -457,7 +510,7 @@ __device__ __forceinline__ void step_full_node(
   }
 }
 
-template <bool GEOM, bool NT>
+template <bool GEOM, bool NT, bool LIN>
 __global__ void __launch_bounds__(kBlock)
     k_ref_step_full(Grid g, const float* __restrict__ A, float* __restrict__ B,
                     Relax tau, FlowSpec flow, const uint32_t* __restrict__ cls,
@@ -465,10 +518,11 @@ __global__ void __launch_bounds__(kBlock)
                     const int32_t* __restrict__ rxbase,
                     const float* __restrict__ gsum, int rev,
                     const float* __restrict__ partials, int npart,
-                    float* __restrict__ force) {
+                    float* __restrict__ force, PullOffsets po) {
   const uint32_t blk = rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
   const uint32_t n = blk * kBlock + threadIdx.x;
-  if (n < g.n) step_full_node<GEOM, NT>(g, A, B, tau, flow, cls, inmask, rxbase, gsum, n);
+  if (n < g.n)
+    step_full_node<GEOM, NT, LIN>(g, A, B, tau, flow, cls, inmask, rxbase, gsum, po, n);
   // the gather pass's force partials are complete when this kernel starts:
   // the first workgroup dispatched adds them to the force register (its
   // extra work overlaps the rest of the grid)
@@ -588,18 +642,37 @@ int launch_ref_step_full(const Grid& g, const float* A, float* B, float tau,
   const bool nt = streaming_stores(2ull * 27 * sizeof(float) * g.n);
   const int r = nt ? rev : 0;
   const Relax rx = make_relax(tau);
-  if (cls && nt)
-    hipLaunchKernelGGL((k_ref_step_full<true, true>), dim3(nb), dim3(kBlock), 0, s, g, A, B, rx,
-                       flow, cls, inmask, rxbase, gsum, r, partials, npart, fsum);
-  else if (cls)
-    hipLaunchKernelGGL((k_ref_step_full<true, false>), dim3(nb), dim3(kBlock), 0, s, g, A, B, rx,
-                       flow, cls, inmask, rxbase, gsum, r, partials, npart, fsum);
-  else if (nt)
-    hipLaunchKernelGGL((k_ref_step_full<false, true>), dim3(nb), dim3(kBlock), 0, s, g, A, B, rx,
-                       flow, nullptr, nullptr, nullptr, nullptr, r, nullptr, 0, nullptr);
-  else
-    hipLaunchKernelGGL((k_ref_step_full<false, false>), dim3(nb), dim3(kBlock), 0, s, g, A, B, rx,
-                       flow, nullptr, nullptr, nullptr, nullptr, r, nullptr, 0, nullptr);
+  // the linear pull: nz a multiple of 64 (a node's y and x neighbours sit
+  // at its place in their tiles) and population offsets below 2^31 (the
+  // offset that reads 0 lies past the array)
+  const bool lin = g.nz % kTile == 0 && 27ull * 4ull * g.n < (1ull << 31);
+  PullOffsets po{};
+  if (lin) {
+    const int64_t plane = (int64_t)g.ny * g.nz;
+    for (int j = 0; j < 27; ++j) {
+      const int64_t dt = -((int64_t)L27::E[j][0] * plane + (int64_t)L27::E[j][1] * g.nz) / kTile;
+      po.b[j] = (int32_t)(4 * (dt * 27 * kTile + (int64_t)j * kTile - L27::E[j][2]));
+    }
+  }
+  auto go = [&](auto kernel, bool geom) {
+    if (geom)
+      hipLaunchKernelGGL(kernel, dim3(nb), dim3(kBlock), 0, s, g, A, B, rx, flow, cls, inmask,
+                         rxbase, gsum, r, partials, npart, fsum, po);
+    else
+      hipLaunchKernelGGL(kernel, dim3(nb), dim3(kBlock), 0, s, g, A, B, rx, flow, nullptr,
+                         nullptr, nullptr, nullptr, r, nullptr, 0, nullptr, po);
+  };
+  if (lin) {
+    if (cls && nt) go(k_ref_step_full<true, true, true>, true);
+    else if (cls) go(k_ref_step_full<true, false, true>, true);
+    else if (nt) go(k_ref_step_full<false, true, true>, false);
+    else go(k_ref_step_full<false, false, true>, false);
+  } else {
+    if (cls && nt) go(k_ref_step_full<true, true, false>, true);
+    else if (cls) go(k_ref_step_full<true, false, false>, true);
+    else if (nt) go(k_ref_step_full<false, true, false>, false);
+    else go(k_ref_step_full<false, false, false>, false);
+  }
   LBM_HIP(hipGetLastError());
   return LBM_OK;
 }

@@ -308,6 +308,40 @@ def test_reference_staged_equals_fused_full_step(data_dir, dims, spacing,
     assert rel_err(outs[1][1], outs[0][1]) <= 1e-6
 
 
+@pytest.mark.parametrize("dims", [(5, 3, 128), (4, 6, 64)])
+def test_reference_fused_rows_of_whole_tiles_vs_staged_and_oracle(dims):
+    # z-rows a multiple of 64 nodes long: the fused step reads each
+    # direction at the node's own offset plus a constant (a z-source across
+    # a tile's end adds one tile), with sources outside the box read as 0;
+    # no flagged node here (the angled cube at 80x72x64 above has them)
+    tau, steps = 0.8, 12
+    say(f"case: reference fused step, rows of whole tiles {dims} x {steps}")
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(11)
+    f = (0.001 * (1 + 0.2 * rng.standard_normal(27 * n))).astype(np.float32)
+    flow = np.full(27, 0.0005, np.float32)
+    outs = []
+    for staged in (True, False):
+        with ref_ctx(dims, tau=tau) as c:
+            c.set_populations(f)
+            c.set_flow(capi.FLOW_PLANE, [2, 1], flow)
+            for _ in range(steps):
+                if staged:
+                    c.stream()
+                    c.collide(tau)
+                    c.flow()
+                else:
+                    c.step_reference(1, False)
+            outs.append(c.populations())
+    none = (np.zeros(0, np.int32), np.zeros(0, np.uint32),
+            np.zeros(0, np.float32))
+    o = O.ref_run_main(dims, f, np.zeros(n, np.uint8), none, tau,
+                       O.FLOW_PLANE, [2, 1], flow, np.zeros(3, np.float32),
+                       steps)
+    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[1], o)
+
+
 def test_reference_long_reflection_groups_vs_staged_and_oracle():
     # Records that send many of their 80 slots into a few directions: the
     # (receiver, direction) groups of the fused step's gather hold up to 42

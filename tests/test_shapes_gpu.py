@@ -80,6 +80,12 @@ CASES = [
     (19, (6, 5, 70), (1, 0, 0), 1, False, True, F16),
     (19, (7, 4, 66), (1, 0, 1), 0, True, False, BF16),
     (19, (5, 6, 40), (1, 1, 1), None, False, True, F16),
+    # FP32 z-rows of whole tiles (a wavefront is one tile of one row): D3Q27
+    # with a moving wall, periodic y, periodic z over three tiles, walls
+    (27, (4, 3, 128), (1, 0, 0), 1, True, False, F32),
+    (19, (3, 5, 128), (0, 1, 0), 0, False, True, F32),
+    (19, (2, 3, 192), (0, 0, 1), None, False, True, F32),
+    (19, (3, 4, 64), (0, 0, 0), 1, True, True, F32),
     # axes of one node: periodic (the node is its own neighbour) and
     # walled (every link along the axis bounces)
     (9, (1, 40, 1), (1, 1, 0), None, False, True, F16),

@@ -160,6 +160,7 @@ __device__ __forceinline__ float div_tau(float x, const Relax& r) {
 //    (ct, usq are floats: at most 2 + 24, 4 + 24, 28 + 24 and 2 + 24
 //    significant bits), so each product-then-add is one FMA with the same
 //    single rounding; the additions keep the source's order;
+//  - opposite directions share e.u in double and 4.5 e.u, negated (exact);
 //  - the division by tau is div_tau.
 __device__ __forceinline__ void collide27(float f[27], const Relax& r) {
   float density = 0.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f;
@@ -177,8 +178,23 @@ __device__ __forceinline__ void collide27(float f[27], const Relax& r) {
   }
   const float usq = (ux * ux) + (uy * uy) + (uz * uz);
   const double m15usq = -1.5 * (double)usq;
+  // e.u of direction i in double, and 4.5 e.u: the opposite direction's
+  // are their exact negatives (its sum has every term negated, and
+  // rounding is symmetric), so a pair shares one conversion and one
+  // product; each direction's arithmetic is unchanged
+  auto relax = [&](int i, double c, double p) {
+    const double w = i == 0 ? 8.0 / 27.0
+                            : (i < 7 ? 2.0 / 27.0 : (i < 19 ? 1.0 / 54.0 : 1.0 / 216.0));
+    // ((1 + 3 ct) + (4.5 ct) ct) - 1.5 usq, each step rounded once
+    const double t1 = fma(3.0, c, 1.0);
+    const double t2 = fma(p, c, t1);
+    const double t3 = t2 + m15usq;
+    const float eq = (float)((w * (double)density) * t3);
+    f[i] = f[i] + div_tau(eq - f[i], r);
+  };
 #pragma unroll
   for (int i = 0; i < 27; ++i) {
+    if (L27::INV[i] < i) continue;
     float ct = 0.0f;
     bool first = true;
     if (L27::E[i][0]) {
@@ -194,15 +210,10 @@ __device__ __forceinline__ void collide27(float f[27], const Relax& r) {
       const float t = L27::E[i][2] > 0 ? uz : -uz;
       ct = first ? t : ct + t;
     }
-    const double w = i == 0 ? 8.0 / 27.0
-                            : (i < 7 ? 2.0 / 27.0 : (i < 19 ? 1.0 / 54.0 : 1.0 / 216.0));
     const double c = (double)ct;
-    // ((1 + 3 ct) + (4.5 ct) ct) - 1.5 usq, each step rounded once
-    const double t1 = fma(3.0, c, 1.0);
-    const double t2 = fma(4.5 * c, c, t1);
-    const double t3 = t2 + m15usq;
-    const float eq = (float)((w * (double)density) * t3);
-    f[i] = f[i] + div_tau(eq - f[i], r);
+    const double p = 4.5 * c;
+    relax(i, c, p);
+    if (L27::INV[i] != i) relax(L27::INV[i], -c, -p);
   }
 }
 

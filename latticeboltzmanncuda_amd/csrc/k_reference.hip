@@ -474,7 +474,7 @@ __device__ __forceinline__ void step_full_node(
     const Grid& g, const float* __restrict__ A, float* __restrict__ B, const Relax& tau,
     const FlowSpec& flow, const uint32_t* __restrict__ cls, const uint32_t* __restrict__ inmask,
     const int32_t* __restrict__ rxbase, const float* __restrict__ gsum, const PullOffsets& po,
-    uint32_t n) {
+    const uint32_t* __restrict__ keep, uint32_t n) {
   uint32_t ux, uy, uz;
   decode(g, n, ux, uy, uz);
   const int x = (int)ux, y = (int)uy, z = (int)uz;
@@ -512,12 +512,34 @@ __device__ __forceinline__ void step_full_node(
 #pragma unroll
     for (int i = 0; i < 27; ++i) f[i] = f[i] + flow.induced[i];
   }
+  if constexpr (GEOM && NT && LIN) {
+    // tiles the next gather reads (around reflecting nodes) are stored with
+    // the ordinary policy, so that they stay in the caches for it, the
+    // others non-temporally; the bit is a wavefront constant (a wavefront
+    // is one tile).  Buffer stores carry the policy as an immediate (the
+    // compiler would merge two branches of plain stores, hint and all).
+    const auto rb = buffer_rsrc(B, 27u * 4u * g.n);
+    const uint32_t own = pidx<27>(n, 0) * 4u;
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(n) >> kTileShift;
+    if (((keep[tile >> 5] >> (tile & 31u)) & 1u) == 0u) {
 #pragma unroll
-  for (int i = 0; i < 27; ++i) {
-    if (NT)
-      __builtin_nontemporal_store(f[i], &B[pidx<27>(n, i)]);
-    else
-      B[pidx<27>(n, i)] = f[i];
+      for (int i = 0; i < 27; ++i)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, f[i]), rb,
+                                              (int)(own + (uint32_t)i * kTile * 4u), 0, 2);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 27; ++i)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, f[i]), rb,
+                                              (int)(own + (uint32_t)i * kTile * 4u), 0, 0);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 27; ++i) {
+      if (NT)
+        __builtin_nontemporal_store(f[i], &B[pidx<27>(n, i)]);
+      else
+        B[pidx<27>(n, i)] = f[i];
+    }
   }
 }
 
@@ -529,11 +551,12 @@ __global__ void __launch_bounds__(kBlock)
                     const int32_t* __restrict__ rxbase,
                     const float* __restrict__ gsum, int rev,
                     const float* __restrict__ partials, int npart,
-                    float* __restrict__ force, PullOffsets po) {
+                    float* __restrict__ force, PullOffsets po,
+                    const uint32_t* __restrict__ keep) {
   const uint32_t blk = rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
   const uint32_t n = blk * kBlock + threadIdx.x;
   if (n < g.n)
-    step_full_node<GEOM, NT, LIN>(g, A, B, tau, flow, cls, inmask, rxbase, gsum, po, n);
+    step_full_node<GEOM, NT, LIN>(g, A, B, tau, flow, cls, inmask, rxbase, gsum, po, keep, n);
   // the gather pass's force partials are complete when this kernel starts:
   // the first workgroup dispatched adds them to the force register (its
   // extra work overlaps the rest of the grid)
@@ -635,6 +658,7 @@ int launch_ref_step_full(const Grid& g, const float* A, float* B, float tau,
                          const uint32_t* inmask, const int32_t* rxbase,
                          int64_t ngrp, const uint32_t* gsrc, const uint32_t* gbeg,
                          const uint32_t* gid, float* gsum, int64_t nent, const uint32_t* eidx,
+                         const uint32_t* keep,
                          const float* ew, const uint8_t* es, float* partials,
                          float* force, int rev, hipStream_t s) {
   const unsigned nb = blocks_for(g.n);
@@ -668,10 +692,10 @@ int launch_ref_step_full(const Grid& g, const float* A, float* B, float tau,
   auto go = [&](auto kernel, bool geom) {
     if (geom)
       hipLaunchKernelGGL(kernel, dim3(nb), dim3(kBlock), 0, s, g, A, B, rx, flow, cls, inmask,
-                         rxbase, gsum, r, partials, npart, fsum, po);
+                         rxbase, gsum, r, partials, npart, fsum, po, keep);
     else
       hipLaunchKernelGGL(kernel, dim3(nb), dim3(kBlock), 0, s, g, A, B, rx, flow, nullptr,
-                         nullptr, nullptr, nullptr, r, nullptr, 0, nullptr, po);
+                         nullptr, nullptr, nullptr, r, nullptr, 0, nullptr, po, nullptr);
   };
   if (lin) {
     if (cls && nt) go(k_ref_step_full<true, true, true>, true);

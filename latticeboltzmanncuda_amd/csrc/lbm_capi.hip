@@ -297,6 +297,8 @@ int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
   const size_t words = (size_t)((N + 15) / 16);
   const size_t ng = (size_t)std::max<int64_t>(c->ref_ngrp, 1), ne = (size_t)std::max<int64_t>(c->ref_nent, 1);
   dev_free(c, (void**)&c->ref_cls, sizeof(uint32_t) * words);
+  const size_t kwords = (size_t)((N + 64 * 32 - 1) / (64 * 32));
+  dev_free(c, (void**)&c->ref_keep, sizeof(uint32_t) * kwords);
   dev_free(c, (void**)&c->ref_inmask, sizeof(uint32_t) * (size_t)N);
   dev_free(c, (void**)&c->ref_rxbase, sizeof(int32_t) * (size_t)N);
   dev_free(c, (void**)&c->ref_gsrc, sizeof(uint32_t) * ng);
@@ -415,7 +417,22 @@ int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
   c->ref_partials_cap = ref_gather_partials(c->ref_ngrp);
   const size_t ng2 = std::max<size_t>(gsrc.size(), 1), ne2 = std::max<size_t>(eidx.size(), 1);
   int rc;
+  // tiles within one node of a reflecting node: the gather reads them
+  std::vector<uint32_t> keep(kwords, 0u);
+  for (int64_t n = 0; n < N; ++n) {
+    if (!(c->host_flags[n] & LBM_FLAG_REFLECTED) || (c->nrec > 0 ? rec_of[n] : -1) < 0) continue;
+    const int64_t x = n / plane, y = (n % plane) / g.nz, z = n % g.nz;
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dz = -1; dz <= 1; ++dz) {
+          const int64_t a = x + dx, b = y + dy, cz = z + dz;
+          if (a < 0 || b < 0 || cz < 0 || a >= g.nx || b >= g.ny || cz >= g.nz) continue;
+          const uint64_t tile = (uint64_t)at(a, b, cz) / 64u;
+          keep[tile >> 5] |= 1u << (tile & 31u);
+        }
+  }
   if ((rc = dev_alloc(c, (void**)&c->ref_cls, sizeof(uint32_t) * words, "hipMalloc(ref classes)")) ||
+      (rc = dev_alloc(c, (void**)&c->ref_keep, sizeof(uint32_t) * kwords, "hipMalloc(ref tiles)")) ||
       (rc = dev_alloc(c, (void**)&c->ref_inmask, sizeof(uint32_t) * (size_t)N, "hipMalloc(ref in-masks)")) ||
       (rc = dev_alloc(c, (void**)&c->ref_rxbase, sizeof(int32_t) * (size_t)N, "hipMalloc(ref receivers)")) ||
       (rc = dev_alloc(c, (void**)&c->ref_gsrc, sizeof(uint32_t) * ng2, "hipMalloc(ref groups)")) ||
@@ -429,6 +446,7 @@ int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
                       "hipMalloc(ref force partials)")))
     return rc;
   if ((rc = h2d(c, c->ref_cls, cls.data(), sizeof(uint32_t) * words)) ||
+      (rc = h2d(c, c->ref_keep, keep.data(), sizeof(uint32_t) * kwords)) ||
       (rc = h2d(c, c->ref_inmask, inmask.data(), sizeof(uint32_t) * (size_t)N)) ||
       (rc = h2d(c, c->ref_rxbase, base.data(), sizeof(int32_t) * (size_t)N)))
     return rc;
@@ -536,7 +554,7 @@ int step_reference_full(lbm_ctx* c, int n, int with_force) {
     if ((rc = launch_ref_step_full(c->grid, (const float*)c->f[c->cur], (float*)c->f[c->cur ^ 1],
                                    c->cfg.tau, c->flow, geom ? c->ref_cls : nullptr,
                                    c->ref_inmask, c->ref_rxbase, c->ref_ngrp, c->ref_gsrc,
-                                   c->ref_gbeg, c->ref_gid, c->ref_gsum, c->ref_nent, c->ref_eidx, c->ref_ew,
+                                   c->ref_gbeg, c->ref_gid, c->ref_gsum, c->ref_nent, c->ref_eidx, c->ref_keep, c->ref_ew,
                                    c->ref_es, geom && with_force ? c->ref_partials : nullptr,
                                    geom && with_force ? c->force : nullptr, c->cur, c->stream)))
       return rc;
@@ -694,7 +712,7 @@ int lbm_destroy(lbm_ctx* c) {
                   (void*)c->special, (void*)c->special_rec,
                   (void*)c->flow_mask, (void*)c->force,
                   (void*)c->force_partials, (void*)c->red, (void*)c->red_out,
-                  (void*)c->ref_cls, (void*)c->ref_inmask, (void*)c->ref_rxbase,
+                  (void*)c->ref_cls, (void*)c->ref_keep, (void*)c->ref_inmask, (void*)c->ref_rxbase,
                   (void*)c->ref_gsrc, (void*)c->ref_gbeg, (void*)c->ref_gid, (void*)c->ref_gsum,
                   (void*)c->ref_eidx, (void*)c->ref_ew, (void*)c->ref_es,
                   (void*)c->ref_partials, c->stage})

@@ -178,6 +178,7 @@ int launch_ref_step_full(const Grid& g, const float* A, float* B, float tau,
                          const uint32_t* inmask, const int32_t* rxbase,
                          int64_t ngrp, const uint32_t* gsrc, const uint32_t* gbeg,
                          const uint32_t* gid, float* gsum, int64_t nent, const uint32_t* eidx,
+                         const uint32_t* keep,
                          const float* ew, const uint8_t* es, float* partials,
                          float* force, int rev, hipStream_t s);
 int ref_gather_partials(int64_t ngrp);
@@ -244,6 +245,7 @@ struct lbm_ctx {
   // then takes that sum as f_j(t)
   bool ref_geom = false;           // any flagged node
   uint32_t* ref_cls = nullptr;     // 2-bit: 0 plain, 1 receiver, 2 internal
+  uint32_t* ref_keep = nullptr;    // bit per tile: the gather reads it (cached stores)
   uint32_t* ref_inmask = nullptr;  // per node: bit j = a reflected link arrives in direction j
   int32_t* ref_rxbase = nullptr;   // per node: first group of a receiver (groups ascend in j)
   int64_t ref_ngrp = 0;            // groups (t, j)

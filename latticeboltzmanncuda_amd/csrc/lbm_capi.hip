@@ -417,14 +417,18 @@ int rebuild_receivers(lbm_ctx* c, const std::vector<int32_t>& rec_of) {
   c->ref_partials_cap = ref_gather_partials(c->ref_ngrp);
   const size_t ng2 = std::max<size_t>(gsrc.size(), 1), ne2 = std::max<size_t>(eidx.size(), 1);
   int rc;
-  // tiles within one node of a reflecting node: the gather reads them
+  // tiles within two nodes of a reflecting node: the gather reads those
+  // within one, and the fused pass all of them (on the 128^3 angled cube a
+  // reach of 2 to 8 nodes gave 85.0-85.1 us per step, 1 node 85.6, 16
+  // nodes 85.6-85.7: the caches start to lose them)
+  constexpr int kKeepReach = 2;
   std::vector<uint32_t> keep(kwords, 0u);
   for (int64_t n = 0; n < N; ++n) {
     if (!(c->host_flags[n] & LBM_FLAG_REFLECTED) || (c->nrec > 0 ? rec_of[n] : -1) < 0) continue;
     const int64_t x = n / plane, y = (n % plane) / g.nz, z = n % g.nz;
-    for (int dx = -1; dx <= 1; ++dx)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dz = -1; dz <= 1; ++dz) {
+    for (int dx = -kKeepReach; dx <= kKeepReach; ++dx)
+      for (int dy = -kKeepReach; dy <= kKeepReach; ++dy)
+        for (int dz = -kKeepReach; dz <= kKeepReach; ++dz) {
           const int64_t a = x + dx, b = y + dy, cz = z + dz;
           if (a < 0 || b < 0 || cz < 0 || a >= g.nx || b >= g.ny || cz >= g.nz) continue;
           const uint64_t tile = (uint64_t)at(a, b, cz) / 64u;

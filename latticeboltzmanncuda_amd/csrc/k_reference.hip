@@ -513,9 +513,9 @@ __device__ __forceinline__ void step_full_node(
     for (int i = 0; i < 27; ++i) f[i] = f[i] + flow.induced[i];
   }
   if constexpr (GEOM && NT && LIN) {
-    // tiles the next gather reads (around reflecting nodes) are stored with
-    // the ordinary policy, so that they stay in the caches for it, the
-    // others non-temporally; the bit is a wavefront constant (a wavefront
+    // tiles around reflecting nodes (which the next step's gather and this
+    // pass read) are stored with the ordinary policy, so that they stay in
+    // the caches for them, the others non-temporally; the bit is a wavefront constant (a wavefront
     // is one tile).  Buffer stores carry the policy as an immediate (the
     // compiler would merge two branches of plain stores, hint and all).
     const auto rb = buffer_rsrc(B, 27u * 4u * g.n);
